@@ -1,0 +1,10 @@
+"""4dzb_amd — MI355X (gfx950) soft-shadow visibility path of galaxy-wolf/4DZB.
+
+The product is the C-ABI library ``libfd4d.so`` (include/fd4d.h) built from
+``csrc/``; ``fd4d`` is its ctypes binding and ``workloads`` the deterministic
+input generators.  Import this package with
+``importlib.import_module("4dzb_amd")`` (the name starts with a digit).
+"""
+from . import fd4d, workloads  # noqa: F401
+
+__all__ = ["fd4d", "workloads"]

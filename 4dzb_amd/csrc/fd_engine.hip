@@ -1,0 +1,527 @@
+// Host side of libfd4d: the context that replaces the reference's process
+// globals (FDc/FourD.cu:26-135), the light set-up (FD::FDsetAreaLight,
+// FD/api.cpp:303-400), workspace management and the C ABI of include/fd4d.h.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fd4d.h"
+#include "fd_kernels.h"
+#include "fd_types.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Light set-up on the host (float arithmetic in the reference's order;
+// compiled with -ffp-contract=off).  glm 0.9.8 semantics for lookAt,
+// infinitePerspective (fovy in radians) and mat4 products
+// (4DZB/glm/gtc/matrix_transform.inl:378-393,521-546,
+// 4DZB/glm/detail/type_mat4x4.inl:595-611).
+// ---------------------------------------------------------------------------
+struct V3 { float x, y, z; };
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 scl(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float dotp(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 crs(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline V3 nrmz(V3 v) { float inv = 1.0f / std::sqrt(dotp(v, v)); return scl(v, inv); }
+
+void light_setup(const fd_area_light& d, FdLight& L) {
+    std::memset(&L, 0, sizeof(L));
+    const int W = d.res_w, H = d.res_h;
+    V3 pos{d.position[0], d.position[1], d.position[2]};
+    V3 view = nrmz({d.view_dir[0], d.view_dir[1], d.view_dir[2]});
+    V3 up = nrmz({d.up_dir[0], d.up_dir[1], d.up_dir[2]});
+    V3 right = nrmz(crs(view, up));
+    V3 top = nrmz(crs(right, view));
+    V3 xd = scl(right, d.right_radius * -2.0f / (float)W);
+    V3 yd = scl(top, d.top_radius * -2.0f / (float)H);
+    const float xf = ((float)W - 1.0f) / (float)W;
+    const float yf = ((float)H - 1.0f) / (float)H;
+    V3 nright = scl(right, -1.0f), ntop = scl(top, -1.0f);
+    V3 ur = add(add(pos, scl(scl(right, xf), d.right_radius)), scl(scl(top, yf), d.top_radius));
+    V3 ul = add(add(pos, scl(scl(nright, xf), d.right_radius)), scl(scl(top, yf), d.top_radius));
+    V3 dl = add(add(pos, scl(scl(nright, xf), d.right_radius)), scl(scl(ntop, yf), d.top_radius));
+    V3 dr = add(add(pos, scl(scl(right, xf), d.right_radius)), scl(scl(ntop, yf), d.top_radius));
+    // view matrix at the up-right sample, looking along view_dir
+    V3 center = add(ur, scl(view, 10.0f));
+    V3 f = nrmz(sub(center, ur));
+    V3 s = nrmz(crs(f, up));
+    V3 u = crs(s, f);
+    float MV[4][4] = {{s.x, u.x, -f.x, 0.0f}, {s.y, u.y, -f.y, 0.0f}, {s.z, u.z, -f.z, 0.0f},
+                      {-dotp(s, ur), -dotp(u, ur), dotp(f, ur), 1.0f}};
+    // infinite perspective, fovy = 120 (radians, glm 0.9.8), aspect 1, near 0.1
+    const float zNear = 0.1f;
+    const float range = std::tan(120.0f / 2.0f) * zNear;
+    const float left = -range * 1.0f, rightp = range * 1.0f, bottom = -range, topp = range;
+    float P[4][4] = {};
+    P[0][0] = (2.0f * zNear) / (rightp - left);
+    P[1][1] = (2.0f * zNear) / (topp - bottom);
+    P[2][2] = -1.0f;
+    P[2][3] = -1.0f;
+    P[3][2] = -2.0f * zNear;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i)
+            L.m[4 * j + i] = ((P[0][i] * MV[j][0] + P[1][i] * MV[j][1]) + P[2][i] * MV[j][2]) + P[3][i] * MV[j][3];
+    const V3* src[6] = {&xd, &yd, &ur, &ul, &dl, &dr};
+    float* dst[6] = {L.xd, L.yd, L.ur, L.ul, L.dl, L.dr};
+    for (int k = 0; k < 6; ++k) { dst[k][0] = src[k]->x; dst[k][1] = src[k]->y; dst[k][2] = src[k]->z; }
+    L.F[0] = d.right_radius * 2.0f * xf * P[0][0] * 0.5f;
+    L.F[1] = d.top_radius * 2.0f * yf * P[1][1] * 0.5f;
+    L.res = W;
+    L.resNum = W * H;
+    L.wpr = (W <= 32) ? 1 : 2;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t want = bytes ? bytes : 256;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+struct Model {
+    const float* verts;
+    int vstride;
+    int nverts;
+    const int* idx;
+    int ntris;
+};
+
+}  // namespace
+
+struct fd_ctx {
+    int device = 0;
+    // configuration (reference defaults: FDc/FourD.cu:61-62, constants.h:13)
+    int cull = 1;
+    float eps = 1e-6f;
+    int gridLog2 = 4;
+    size_t maxTris = 0, maxVerts = 0;
+    int W = 0, H = 0;
+    std::vector<Model> models;
+    const float4* pos = nullptr;
+    const float4* nrm = nullptr;
+    float* shadow = nullptr;
+    uint32_t* masks = nullptr;
+    bool lightSet = false;
+    fd_area_light lightDesc{};
+    FdLight light{};
+    unsigned long long pairCap = 64ull << 20;
+    // workspace
+    DevBuf st, rec, rxKey, histR, sortedId, sortedRec, binStat, triAABB, tri, triCount, triRange, histT, sortedTri,
+        tables, scanPart, scratchMasks;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    FdArgs last{};
+    bool haveLast = false;
+    bool timed = false;
+};
+
+namespace {
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return FD_OK;
+    std::fprintf(stderr, "fd4d: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
+    return FD_ERR_HIP;
+}
+
+// Size the workspace for the current view / scene / grid and fill FdArgs.
+int prepare(fd_ctx* c, FdArgs& a) {
+    if (c->models.empty() || !c->pos || !c->nrm || !c->shadow || c->W <= 0 || c->H <= 0 || !c->lightSet)
+        return FD_ERR_STATE;
+    if (c->lightDesc.res_w != c->lightDesc.res_h || c->lightDesc.res_w < 1 || c->lightDesc.res_w > 64)
+        return FD_ERR_UNSUPPORTED_LIGHT;
+    const Model& md = c->models[0];
+    std::memset(&a, 0, sizeof(a));
+    a.L = c->light;
+    a.eps = c->eps;
+    a.cull = c->cull;
+    a.gridLog2 = c->gridLog2;
+    a.verts = md.verts;
+    a.vstride = md.vstride;
+    a.nverts = md.nverts;
+    a.idx = md.idx;
+    a.ntris = md.ntris;
+    a.pos = c->pos;
+    a.nrm = c->nrm;
+    a.W = c->W;
+    a.H = c->H;
+    a.WH = c->W * c->H;
+    a.shadow = c->shadow;
+    const int nb = 1 << (2 * c->gridLog2);
+    const long WH = a.WH;
+    const int R = c->light.res, wpr = c->light.wpr;
+    a.nblkR = cdiv(WH, FD_RXB);
+    a.nblkT = cdiv(std::max(md.ntris, 1), FD_TRB);
+    hipError_t e = hipSuccess;
+    auto need = [&](DevBuf& b, size_t bytes) { if (e == hipSuccess) e = b.ensure(bytes); return b.p; };
+    a.st = (FdState*)need(c->st, sizeof(FdState));
+    a.rec = (float4*)need(c->rec, WH * sizeof(float4));
+    a.rxKey = (uint16_t*)need(c->rxKey, WH * sizeof(uint16_t));
+    const long nR = (long)(nb + 1) * a.nblkR, nT = (long)nb * a.nblkT;
+    a.histR = (uint32_t*)need(c->histR, nR * sizeof(uint32_t));
+    a.sortedId = (int*)need(c->sortedId, WH * sizeof(int));
+    a.sortedRec = (FdRxRec*)need(c->sortedRec, WH * sizeof(FdRxRec));
+    a.binStat = (uint32_t*)need(c->binStat, (size_t)nb * 5 * sizeof(uint32_t));
+    const size_t T = (size_t)std::max(md.ntris, 1);
+    a.triAABB = (float4*)need(c->triAABB, 2 * T * sizeof(float4));
+    a.tri = (FdTri*)need(c->tri, T * sizeof(FdTri));
+    a.triCount = (int*)need(c->triCount, T * sizeof(int));
+    a.triRange = (uint2*)need(c->triRange, T * sizeof(uint2));
+    a.histT = (uint32_t*)need(c->histT, nT * sizeof(uint32_t));
+    a.pairCap = c->pairCap;
+    a.sortedTri = (int*)need(c->sortedTri, (size_t)c->pairCap * sizeof(int));
+    int* tb = (int*)need(c->tables, (size_t)(5 * (nb + 2)) * sizeof(int));
+    a.rxStart = tb;
+    a.rxEnd = tb + (nb + 2);
+    a.triStart = tb + 2 * (nb + 2);
+    a.triEnd = tb + 3 * (nb + 2);
+    a.chunkStart = tb + 4 * (nb + 2);
+    const long np = std::max(fdk::scan_partials_needed(nR), fdk::scan_partials_needed(nT)) + 1;
+    a.scanPart = (uint64_t*)need(c->scanPart, np * sizeof(uint64_t));
+    if (c->masks) a.masks = c->masks;
+    else a.masks = (uint32_t*)need(c->scratchMasks, (size_t)WH * R * wpr * sizeof(uint32_t));
+    return hip_status(e);
+}
+
+void fill_stats(const FdState& s, fd_stats* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->exit_code = s.status;
+    o->valid_receivers = s.validSamples;
+    o->front_receivers = s.validRect;
+    o->valid_triangles = s.validTris;
+    o->pairs = s.pairs;
+    o->key_out_of_range = s.keyOutOfRange;
+    o->work_items = s.totalChunks;
+    o->light_plane_begin[0] = s.B[0];
+    o->light_plane_begin[1] = s.B[1];
+    o->light_plane_end[0] = s.E[0];
+    o->light_plane_end[1] = s.E[1];
+    o->far_plane = s.f;
+}
+
+struct M16 { float m[16]; };
+
+__global__ void k_move_model(float* vbo, int vstride, int nv, int nthreads, M16 mm) {
+    const float* m = mm.m;
+    // cudaModeTransform's transform kernel (FDc/moveModel.inl:6-30)
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < (unsigned)nv; t += nthreads) {
+        float* p = vbo + (size_t)t * vstride;
+        const float x = p[0], y = p[1], z = p[2], w = 1.0f;
+        const float rx = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
+        const float ry = m[1] * x + m[5] * y + m[9] * z + m[13] * w;
+        const float rz = m[2] * x + m[6] * y + m[10] * z + m[14] * w;
+        p[0] = rx;
+        p[1] = ry;
+        p[2] = rz;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int fd_init(fd_ctx** out) {
+    if (!out) return FD_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FD_ERR_NO_DEVICE;
+    fd_ctx* c = new fd_ctx();
+    (void)hipGetDevice(&c->device);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return FD_ERR_HIP;
+    }
+    *out = c;
+    return FD_OK;
+}
+
+int fd_destroy(fd_ctx* c) {
+    if (!c) return FD_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    DevBuf* bufs[] = {&c->st, &c->rec, &c->rxKey, &c->histR, &c->sortedId, &c->sortedRec, &c->binStat,
+                      &c->triAABB, &c->tri, &c->triCount, &c->triRange, &c->histT, &c->sortedTri,
+                      &c->tables, &c->scanPart, &c->scratchMasks};
+    for (DevBuf* b : bufs) b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    return FD_OK;
+}
+
+int fd_set_backface_cull(fd_ctx* c, int enable) {
+    if (!c) return FD_ERR_ARG;
+    c->cull = enable ? 1 : 0;
+    return FD_OK;
+}
+
+int fd_add_model(fd_ctx* c, const float* v, size_t vsz, int nv, const int* idx, int nt) {
+    if (!c || !v || !idx || nv <= 0 || nt < 0 || vsz < 12 || vsz % 4) return FD_ERR_ARG;
+    if (c->maxTris && (size_t)nt > c->maxTris) return FD_ERR_ARG;
+    if (c->maxVerts && (size_t)nv > c->maxVerts) return FD_ERR_ARG;
+    c->models.push_back(Model{v, (int)(vsz / 4), nv, idx, nt});
+    return FD_OK;
+}
+
+int fd_clear_models(fd_ctx* c) {
+    if (!c) return FD_ERR_ARG;
+    c->models.clear();
+    return FD_OK;
+}
+
+int fd_set_sample_buffers(fd_ctx* c, const float* p4, const float* n4) {
+    if (!c || !p4 || !n4) return FD_ERR_ARG;
+    c->pos = (const float4*)p4;
+    c->nrm = (const float4*)n4;
+    return FD_OK;
+}
+
+int fd_set_output_buffers(fd_ctx* c, float* shadow, uint32_t* masks) {
+    if (!c || !shadow) return FD_ERR_ARG;
+    c->shadow = shadow;
+    c->masks = masks;
+    return FD_OK;
+}
+
+int fd_register_res(fd_ctx* c) {
+    if (!c) return FD_ERR_ARG;
+    return (c->pos && c->nrm && c->shadow && !c->models.empty()) ? FD_OK : FD_ERR_STATE;
+}
+
+int fd_unregister_res(fd_ctx* c) {
+    if (!c) return FD_ERR_ARG;
+    c->pos = nullptr;
+    c->nrm = nullptr;
+    c->shadow = nullptr;
+    c->masks = nullptr;
+    return FD_OK;
+}
+
+int fd_set_bin_num(fd_ctx* c, size_t n) {
+    if (!c) return FD_ERR_ARG;
+    int l = 0;
+    while ((size_t(1) << l) < n) ++l;
+    if ((size_t(1) << l) != n || l < 3 || l > 6) return -1;  // the reference's -1 for a bad size
+    c->gridLog2 = l;
+    return FD_OK;
+}
+
+int fd_set_view(fd_ctx* c, int w, int h) {
+    if (!c || w <= 0 || h <= 0 || (long)w * h > (1l << 28)) return FD_ERR_ARG;
+    c->W = w;
+    c->H = h;
+    return FD_OK;
+}
+
+int fd_set_scenes_esp(fd_ctx* c, float eps) {
+    if (!c) return FD_ERR_ARG;
+    c->eps = eps;
+    return FD_OK;
+}
+
+int fd_set_scene_params(fd_ctx* c, size_t maxTris, size_t maxVerts) {
+    if (!c) return FD_ERR_ARG;
+    c->maxTris = maxTris;
+    c->maxVerts = maxVerts;
+    return FD_OK;
+}
+
+int fd_set_area_light(fd_ctx* c, const fd_area_light* d) {
+    if (!c || !d) return FD_ERR_ARG;
+    if (d->res_w != d->res_h || d->res_w < 1 || d->res_w > 64) return FD_ERR_UNSUPPORTED_LIGHT;
+    c->lightDesc = *d;
+    light_setup(*d, c->light);
+    c->lightSet = true;
+    return FD_OK;
+}
+
+int fd_launch_async(fd_ctx* c, void* stream) {
+    if (!c) return FD_ERR_ARG;
+    FdArgs a;
+    int rc = prepare(c, a);
+    if (rc != FD_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = fdk::enqueue_frame(a, st);
+    c->last = a;
+    c->haveLast = true;
+    return hip_status(e);
+}
+
+int fd_get_stats(fd_ctx* c, void* stream, fd_stats* out) {
+    if (!c || !c->haveLast) return FD_ERR_STATE;
+    FdState s;
+    hipError_t e = hipMemcpyAsync(&s, c->last.st, sizeof(s), hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return hip_status(e);
+    if (out) fill_stats(s, out);
+    return s.status == FD_EXIT_CAPACITY ? FD_ERR_CAPACITY : FD_OK;
+}
+
+int fd_launch(fd_ctx* c, void* stream, fd_stats* out) {
+    if (!c) return FD_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    (void)hipEventRecord(c->ev0, st);
+    int rc = fd_launch_async(c, stream);
+    if (rc != FD_OK) return rc;
+    (void)hipEventRecord(c->ev1, st);
+    FdState s;
+    hipError_t e = hipMemcpyAsync(&s, c->last.st, sizeof(s), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_status(e);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (out) {
+        fill_stats(s, out);
+        out->elapsed_ms = ms;
+    }
+    return s.status == FD_EXIT_CAPACITY ? FD_ERR_CAPACITY : FD_OK;
+}
+
+int fd_set_pair_capacity(fd_ctx* c, size_t n) {
+    if (!c || n == 0 || n >= 0x7fffffffull) return FD_ERR_ARG;
+    c->pairCap = n;
+    return FD_OK;
+}
+
+int fd_move_model(fd_ctx* c, const float m[16], void* stream) {
+    if (!c || !m) return FD_ERR_ARG;
+    if (c->models.empty()) return FD_ERR_STATE;
+    const Model& md = c->models[0];
+    M16 mm;
+    std::memcpy(mm.m, m, sizeof(mm.m));
+    const int grid = std::min(65536, std::max(md.nverts, 1));
+    hipLaunchKernelGGL(k_move_model, dim3(grid), dim3(256), 0, (hipStream_t)stream, (float*)md.verts, md.vstride,
+                       md.nverts, grid * 256, mm);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    return hip_status(e);
+}
+
+int fd_mask_words(int res) {
+    if (res < 1 || res > 64) return FD_ERR_ARG;
+    return res * (res <= 32 ? 1 : 2);
+}
+
+const char* fd_status_string(int code) {
+    switch (code) {
+        case FD_OK: return "ok";
+        case FD_ERR_ARG: return "invalid argument";
+        case FD_ERR_STATE: return "missing set-up call";
+        case FD_ERR_HIP: return "HIP runtime error";
+        case FD_ERR_CAPACITY: return "pair arena too small";
+        case FD_ERR_UNSUPPORTED_LIGHT: return "unsupported light grid";
+        case FD_ERR_NO_DEVICE: return "no GPU device";
+        default: return "unknown";
+    }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Stage read-back (verification only; synchronises)
+// ---------------------------------------------------------------------------
+namespace fdk {
+hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st);
+}
+
+extern "C" int fd_debug_read(fd_ctx* c, const char* name, void* dst, size_t maxb, size_t* bytes) {
+    if (!c || !name) return FD_ERR_ARG;
+    if (!c->haveLast) return FD_ERR_STATE;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_status(e);
+    const FdArgs& a = c->last;
+    FdState s;
+    (void)hipMemcpy(&s, a.st, sizeof(s), hipMemcpyDeviceToHost);
+    const int nb = 1 << (2 * a.gridLog2);
+    std::vector<char> out;
+    auto get = [&](const void* src, size_t nbytes) {
+        out.resize(nbytes);
+        if (nbytes) e = hipMemcpy(out.data(), src, nbytes, hipMemcpyDeviceToHost);
+    };
+    const std::string n(name);
+    const bool pipelineRan = s.status == FD_EXIT_OK || s.status == FD_EXIT_NO_PAIRS || s.status == FD_EXIT_NO_VALID_TRI;
+    if (n == "state") {
+        out.resize(sizeof(s));
+        std::memcpy(out.data(), &s, sizeof(s));
+    } else if (n == "light") {
+        out.resize(sizeof(FdLight));
+        std::memcpy(out.data(), &a.L, sizeof(FdLight));
+    } else if (n == "rect") {
+        float* d = nullptr;
+        e = hipMalloc(&d, (size_t)a.WH * 8 * sizeof(float));
+        if (e == hipSuccess) e = fdk::debug_rect(a, d, nullptr);
+        if (e == hipSuccess) get(d, (size_t)a.WH * 8 * sizeof(float));
+        if (d) (void)hipFree(d);
+    } else if (n == "key") {
+        std::vector<uint16_t> k(a.WH);
+        e = hipMemcpy(k.data(), a.rxKey, a.WH * sizeof(uint16_t), hipMemcpyDeviceToHost);
+        out.resize(a.WH * sizeof(int));
+        int* o = (int*)out.data();
+        for (int i = 0; i < a.WH; ++i) o[i] = k[i];
+    } else if (n == "sorted_id") {
+        get(a.sortedId, (size_t)a.WH * sizeof(int));
+    } else if (n == "bin_start" || n == "bin_end" || n == "bin_tri_start" || n == "bin_tri_end") {
+        const bool tri = n.find("tri") != std::string::npos;
+        const int cnt = tri ? nb : nb + 1;
+        std::vector<int> st(cnt), en(cnt);
+        if (pipelineRan && !(tri && s.status != FD_EXIT_OK)) {
+            e = hipMemcpy(st.data(), tri ? a.triStart : a.rxStart, cnt * sizeof(int), hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(en.data(), tri ? a.triEnd : a.rxEnd, cnt * sizeof(int), hipMemcpyDeviceToHost);
+        }
+        // FindStart_kernel leaves absent keys at (0, 0) (binRaster.inl:38-45)
+        for (int k = 0; k < cnt; ++k)
+            if (st[k] == en[k]) st[k] = en[k] = 0;
+        out.resize(cnt * sizeof(int));
+        std::memcpy(out.data(), (n == "bin_start" || n == "bin_tri_start") ? st.data() : en.data(), cnt * sizeof(int));
+    } else if (n == "bin_maxz" || n == "bin_minrange") {
+        std::vector<uint32_t> raw((size_t)nb * 5);
+        e = hipMemcpy(raw.data(), a.binStat, raw.size() * 4, hipMemcpyDeviceToHost);
+        auto dec = [](uint32_t v) {
+            uint32_t u = (v & 0x80000000u) ? (v & 0x7fffffffu) : ~v;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return f;
+        };
+        if (n == "bin_maxz") {
+            out.resize(nb * sizeof(float));
+            float* o = (float*)out.data();
+            for (int b = 0; b < nb; ++b) o[b] = dec(raw[5 * b]);
+        } else {
+            out.resize(nb * 4 * sizeof(float));
+            float* o = (float*)out.data();
+            for (int b = 0; b < nb; ++b)
+                for (int k = 0; k < 4; ++k) o[4 * b + k] = dec(raw[5 * b + 1 + k]);
+        }
+    } else if (n == "tri_aabb") {
+        get(a.triAABB, (size_t)a.ntris * 2 * sizeof(float4));
+    } else if (n == "tri_data") {
+        get(a.tri, (size_t)a.ntris * sizeof(FdTri));
+    } else if (n == "tri_count") {
+        get(a.triCount, (size_t)a.ntris * sizeof(int));
+    } else if (n == "tri_range") {
+        get(a.triRange, (size_t)a.ntris * sizeof(uint2));
+    } else if (n == "pair_tri") {
+        const size_t np = std::min<unsigned long long>(s.pairs, a.pairCap);
+        get(a.sortedTri, np * sizeof(int));
+    } else {
+        return FD_ERR_ARG;
+    }
+    if (e != hipSuccess) return hip_status(e);
+    if (bytes) *bytes = out.size();
+    if (dst && maxb) std::memcpy(dst, out.data(), std::min(maxb, out.size()));
+    return FD_OK;
+}

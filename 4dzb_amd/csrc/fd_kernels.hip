@@ -1,0 +1,1003 @@
+// gfx950 kernels of the soft-shadow visibility path (one frame).
+//
+// Stage map (reference -> kernel):
+//   sampleMaxZcal / triangleMaxZcal / sampleRectangleAABBcal /
+//   verticesRectangleAABBcal + AABBReduce_kernel  -> k_reduce (one pass)
+//   lightPlaneParamCal host part                 -> k_finalize
+//   writeSampleAndBinPair + maxZpreBinReduce      -> k_rx_key
+//   thrust::sort_by_key (receivers) + FindStart    -> k_scan_* + k_rx_scatter
+//   triangleVertexSetup + countPairNumForPerTri    -> k_tri_count
+//   exclusive_scan + writeTriBinPair + sort_by_key -> k_scan_* + k_tri_scatter
+//   findTriangleStartEnd                           -> k_tables
+//   shadowCal_per_sample_kernel                    -> k_occlude
+//
+// Every kernel reads the frame status first, so the whole frame is one
+// fixed launch sequence with no host round trip (graph-capturable).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fd_device.h"
+#include "fd_kernels.h"
+
+using namespace fdk;
+
+namespace {
+
+constexpr int RED_BLOCK = 256;
+constexpr int RED_ITEMS = 4;           // receivers / vertices per thread in k_reduce
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;
+
+// --------------------------------------------------------------------------
+// k_init: reset the frame state (a kernel, not a memset, so that replays of
+// a captured graph start from the same state).
+// --------------------------------------------------------------------------
+__global__ void k_init(FdArgs a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    FdState* s = a.st;
+    const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
+    s->rxZmaxAll = encNegMax;
+    s->rxMin[0] = s->rxMin[1] = s->rxMin[2] = encMax;
+    s->rxMaxA[0] = s->rxMaxA[1] = encNegMax;
+    s->rxMaxZ = encNegMax;
+    s->vxZmax = encNegMax;
+    s->vxMin[0] = s->vxMin[1] = s->vxMin[2] = encMax;
+    s->vxMaxA[0] = s->vxMaxA[1] = encNegMax;
+    s->vxMaxZ = encNegMax;
+    s->validRect = 0;
+    s->status = FD_EXIT_OK;
+    s->validSamples = 0;
+    s->validTris = 0;
+    s->keyOutOfRange = 0;
+    s->capacityHit = 0;
+    s->pairs = 0;
+    s->rxTotal = 0;
+    s->totalChunks = 0;
+}
+
+// Bin statistics reset (maxZ, minRange) for the bins of this frame.
+__global__ void k_init_bins(FdArgs a) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = 1 << (2 * a.gridLog2);
+    if (b >= nb) return;
+    const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
+    a.binStat[5 * b + 0] = encNegMax;  // maxZ
+    a.binStat[5 * b + 1] = encNegMax;  // max of rect min x
+    a.binStat[5 * b + 2] = encNegMax;  // max of rect min y
+    a.binStat[5 * b + 3] = encMax;     // min of rect max x
+    a.binStat[5 * b + 4] = encMax;     // min of rect max y
+}
+
+// Block-level reduction of NB uint32 values with op (0 = min, 1 = max),
+// then one atomic per block into dst.
+template <int NV>
+__device__ void block_reduce_atomic(uint32_t (&v)[NV], const int (&op)[NV], uint32_t* const (&dst)[NV]) {
+    __shared__ uint32_t sh[NV][RED_BLOCK / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        uint32_t r = op[k] ? wave_max_u32(v[k]) : wave_min_u32(v[k]);
+        if (l == 0) sh[k][w] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        uint32_t r = sh[k][0];
+        for (int i = 1; i < RED_BLOCK / 64; ++i) r = op[k] ? max(r, sh[k][i]) : min(r, sh[k][i]);
+        if (op[k]) atomicMax(dst[k], r); else atomicMin(dst[k], r);
+    }
+}
+
+// --------------------------------------------------------------------------
+// k_reduce: one pass over receivers (blocks [0, nbR)) and vertices (blocks
+// [nbR, nbR + nbV)).  Receivers: z for the far plane
+// (Step2_1_samplerRectangleAABB.inl:1-34), back-face test against the four
+// light corners and the light-plane rectangle (:53-137).  The rectangle's max
+// corner x' + F/z - S is reduced as max(x' + F/z) - S after the frame's S is
+// known: rounding of (a - S) is monotone in a, so both give the same float.
+// Vertices: Step2_2_modelRectangleAABB.inl:1-99.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce(FdArgs a, int nbR) {
+    const float* m = a.L.m;
+    const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
+    if ((int)blockIdx.x < nbR) {
+        uint32_t v[7] = {encNegMax, encMax, encMax, encMax, encNegMax, encNegMax, encNegMax};
+        int cnt = 0;
+        const int base = blockIdx.x * RED_BLOCK * RED_ITEMS + threadIdx.x;
+#pragma unroll
+        for (int it = 0; it < RED_ITEMS; ++it) {
+            const int i = base + it * RED_BLOCK;
+            if (i >= a.WH) break;
+            const float4 p = a.pos[i];
+            float4 out = make_float4(0.f, 0.f, 0.f, __uint_as_float(FD_TAG_NONE));
+            if (!(p.x != p.x)) {
+                const float z = getZ(m, p.x, p.y, p.z);
+                if (!(z != z)) v[0] = max(v[0], fenc(z));
+                const float4 n = a.nrm[i];
+                const float* c[4] = {a.L.ur, a.L.ul, a.L.dl, a.L.dr};
+                bool back = true;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    back = back && (dot3(c[k][0] - p.x, c[k][1] - p.y, c[k][2] - p.z, n.x, n.y, n.z) <= 0);
+                if (back) {
+                    out.w = __uint_as_float(FD_TAG_BACK);
+                } else {
+                    float rx, ry, rz;
+                    makeRect(m, p.x, p.y, p.z, rx, ry, rz);
+                    const float ax = rx + a.L.F[0] / rz;
+                    const float ay = ry + a.L.F[1] / rz;
+                    if (!(rx != rx)) v[1] = min(v[1], fenc(rx));
+                    if (!(ry != ry)) v[2] = min(v[2], fenc(ry));
+                    if (!(rz != rz)) { v[3] = min(v[3], fenc(rz)); v[6] = max(v[6], fenc(rz)); }
+                    if (!(ax != ax)) v[4] = max(v[4], fenc(ax));
+                    if (!(ay != ay)) v[5] = max(v[5], fenc(ay));
+                    ++cnt;
+                    out = make_float4(rx, ry, rz, __uint_as_float(FD_TAG_VALID));
+                }
+            }
+            a.rec[i] = out;
+        }
+        FdState* s = a.st;
+        const int op[7] = {1, 0, 0, 0, 1, 1, 1};
+        uint32_t* const dst[7] = {&s->rxZmaxAll, &s->rxMin[0], &s->rxMin[1], &s->rxMin[2],
+                                  &s->rxMaxA[0], &s->rxMaxA[1], &s->rxMaxZ};
+        block_reduce_atomic<7>(v, op, dst);
+        cnt = wave_sum_i32(cnt);
+        if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s->validRect, cnt);
+    } else {
+        uint32_t v[7] = {encNegMax, encMax, encMax, encMax, encNegMax, encNegMax, encNegMax};
+        const int base = (blockIdx.x - nbR) * RED_BLOCK * RED_ITEMS + threadIdx.x;
+#pragma unroll
+        for (int it = 0; it < RED_ITEMS; ++it) {
+            const int vi = base + it * RED_BLOCK;
+            if (vi >= a.nverts) break;
+            const float* p = a.verts + (size_t)vi * a.vstride;
+            const float px = p[0], py = p[1], pz = p[2];
+            float rx, ry, rz;
+            makeRect(m, px, py, pz, rx, ry, rz);
+            const float ax = rx + a.L.F[0] / rz;
+            const float ay = ry + a.L.F[1] / rz;
+            if (!(rz != rz)) { v[0] = max(v[0], fenc(rz)); v[3] = min(v[3], fenc(rz)); v[6] = max(v[6], fenc(rz)); }
+            if (!(rx != rx)) v[1] = min(v[1], fenc(rx));
+            if (!(ry != ry)) v[2] = min(v[2], fenc(ry));
+            if (!(ax != ax)) v[4] = max(v[4], fenc(ax));
+            if (!(ay != ay)) v[5] = max(v[5], fenc(ay));
+        }
+        FdState* s = a.st;
+        const int op[7] = {1, 0, 0, 0, 1, 1, 1};
+        uint32_t* const dst[7] = {&s->vxZmax, &s->vxMin[0], &s->vxMin[1], &s->vxMin[2],
+                                  &s->vxMaxA[0], &s->vxMaxA[1], &s->vxMaxZ};
+        block_reduce_atomic<7>(v, op, dst);
+    }
+}
+
+// --------------------------------------------------------------------------
+// k_finalize: the host part of lightPlaneParamCal
+// (Step2_3_defineFDbuffer.inl:5-111) on the device, with its early exits.
+// --------------------------------------------------------------------------
+__global__ void k_finalize(FdArgs a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    FdState* s = a.st;
+    // sampleMaxZcal / triangleMaxZcal: thrust::reduce(max) seeded with -1
+    const float zs = fmax_(-1.0f, fdec(s->rxZmaxAll));
+    const float zt = fmax_(-1.0f, fdec(s->vxZmax));
+    s->sampleMaxZ = zs;
+    s->triMaxZ = zt;
+    if (zs < 0.0f) { s->status = FD_EXIT_SAMPLE_MAXZ; return; }
+    if (zt < 0.0f) { s->status = FD_EXIT_TRI_MAXZ; return; }
+    const float f = fmax_(zs, zt) + 0.1f;
+    s->f = f;
+    const float Sx = a.L.F[0] / f, Sy = a.L.F[1] / f;
+    s->S[0] = Sx;
+    s->S[1] = Sy;
+    float sa[8] = {fdec(s->rxMin[0]), fdec(s->rxMin[1]), fdec(s->rxMin[2]), 0.0f,
+                   fdec(s->rxMaxA[0]) - Sx, fdec(s->rxMaxA[1]) - Sy, fdec(s->rxMaxZ), 0.0f};
+    float ma[8] = {fdec(s->vxMin[0]), fdec(s->vxMin[1]), fdec(s->vxMin[2]), 0.0f,
+                   fdec(s->vxMaxA[0]) - Sx, fdec(s->vxMaxA[1]) - Sy, fdec(s->vxMaxZ), 0.0f};
+    // the reductions started from +-FD_F32_MAX; (-MAX) - S rounds back to -MAX
+    sa[4] = fmax_(sa[4], -FD_F32_MAX); sa[5] = fmax_(sa[5], -FD_F32_MAX);
+    ma[4] = fmax_(ma[4], -FD_F32_MAX); ma[5] = fmax_(ma[5], -FD_F32_MAX);
+    for (int k = 0; k < 8; ++k) { s->sAABB[k] = sa[k]; s->mAABB[k] = ma[k]; }
+    if (s->validRect == 0) { s->status = FD_EXIT_NO_VALID_SAMPLE; return; }
+    if (ma[0] > ma[4] || ma[1] > ma[5] || sa[0] > sa[4] || sa[1] > sa[5]) { s->status = FD_EXIT_EMPTY_AABB; return; }
+    float B0 = fmax_(sa[0], ma[0]), B1 = fmax_(sa[1], ma[1]);
+    float E0 = fmin_(sa[4], ma[4]), E1 = fmin_(sa[5], ma[5]);
+    s->B[0] = B0; s->B[1] = B1; s->B[2] = ma[2];
+    s->E[0] = E0; s->E[1] = E1; s->E[2] = sa[6];
+    if (B0 >= E0 || B1 >= E1) { s->status = FD_EXIT_EMPTY_LP; return; }
+    const int G = 1 << a.gridLog2;
+    s->G = G;
+    s->log2G = a.gridLog2;
+    s->nb = G * G;
+    s->factor[0] = (float)G / (E0 - B0);
+    s->factor[1] = (float)G / (E1 - B1);
+}
+
+// --------------------------------------------------------------------------
+// Receiver key (writeSampleAndBinPair, Step3_bindSampleToBin.inl:5-59) and
+// the per-bin receiver statistics (maxZpreBinReduce_kernel, :61-104), which
+// are min/max reductions and so are folded into the same pass with atomics.
+// Each block owns RXB consecutive receivers and writes its key histogram.
+// --------------------------------------------------------------------------
+struct RxRect { float bx, by, ex, ey, z; };
+
+__device__ __forceinline__ RxRect rx_rect(const float4 r, const FdState* s, const FdLight& L) {
+    RxRect o;
+    o.bx = r.x;
+    o.by = r.y;
+    o.ex = r.x + L.F[0] / r.z - s->S[0];
+    o.ey = r.y + L.F[1] / r.z - s->S[1];
+    o.z = r.z;
+    return o;
+}
+
+// bin key of a valid receiver, or nb when outside the light plane
+__device__ __forceinline__ int rx_key(const RxRect& q, const FdState* s, int nb, int log2G, bool& oor) {
+    oor = false;
+    const float B0 = s->B[0], B1 = s->B[1], E0 = s->E[0], E1 = s->E[1];
+    if ((q.bx > q.ex || q.by > q.ey) || (q.bx < B0 || q.by < B1 || q.ex > E0 || q.ey > E1)) return nb;
+    const float cx = (q.ex + q.bx) / 2.0f;
+    const float cy = (q.ey + q.by) / 2.0f;
+    const int bx = cvt_sat(floorf((cx - B0) * s->factor[0]));
+    const int by = cvt_sat(floorf((cy - B1) * s->factor[1]));
+    const int k = (int)(((unsigned)by << log2G) | (unsigned)bx);
+    if (k < 0 || k >= nb) { oor = true; return nb; }
+    return k;
+}
+
+__global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
+    extern __shared__ uint32_t hist[];  // nb + 1
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb, log2G = s->log2G;
+    for (int k = threadIdx.x; k <= nb; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    int nvalid = 0, noor = 0;
+    const int base = blockIdx.x * FD_RXB;
+    for (int it = 0; it < FD_RXB / 256; ++it) {
+        const int i = base + it * 256 + threadIdx.x;
+        int key = nb;
+        RxRect q{};
+        if (i < a.WH) {
+            const float4 r = a.rec[i];
+            if (__float_as_uint(r.w) == FD_TAG_VALID) {
+                q = rx_rect(r, s, a.L);
+                bool oor;
+                key = rx_key(q, s, nb, log2G, oor);
+                if (key < nb || oor) ++nvalid;
+                if (oor) ++noor;
+            }
+            a.rxKey[i] = (uint16_t)key;
+            atomicAdd(&hist[key], 1u);
+        }
+        // per-bin statistics: wave-aggregate when the wave shares one key
+        const bool has = (i < a.WH) && key < nb;
+        const uint64_t hasm = __ballot(has);
+        const int k0 = hasm ? uniform(__shfl(key, __ffsll((unsigned long long)hasm) - 1, 64)) : -1;
+        const uint64_t same = __ballot(has && key == k0);
+        uint32_t v0 = has ? fenc(q.z) : 0u, v1 = has ? fenc(q.bx) : 0u, v2 = has ? fenc(q.by) : 0u;
+        uint32_t v3 = has ? fenc(q.ex) : 0xffffffffu, v4 = has ? fenc(q.ey) : 0xffffffffu;
+        // NaN never reaches a bin: a NaN rectangle fails no comparison, but its
+        // centre floors to bin 0; keep NaN out of the ordered encodings.
+        if (has) {
+            if (q.z != q.z) v0 = 0u;
+            if (q.bx != q.bx) v1 = 0u;
+            if (q.by != q.by) v2 = 0u;
+            if (q.ex != q.ex) v3 = 0xffffffffu;
+            if (q.ey != q.ey) v4 = 0xffffffffu;
+        }
+        if (hasm != 0 && same == hasm) {
+            v0 = wave_max_u32(v0); v1 = wave_max_u32(v1); v2 = wave_max_u32(v2);
+            v3 = wave_min_u32(v3); v4 = wave_min_u32(v4);
+            if (lane_id() == 0) {
+                uint32_t* bs = a.binStat + 5 * k0;
+                atomicMax(bs + 0, v0); atomicMax(bs + 1, v1); atomicMax(bs + 2, v2);
+                atomicMin(bs + 3, v3); atomicMin(bs + 4, v4);
+            }
+        } else if (has) {
+            uint32_t* bs = a.binStat + 5 * key;
+            atomicMax(bs + 0, v0); atomicMax(bs + 1, v1); atomicMax(bs + 2, v2);
+            atomicMin(bs + 3, v3); atomicMin(bs + 4, v4);
+        }
+    }
+    nvalid = wave_sum_i32(nvalid);
+    noor = wave_sum_i32(noor);
+    if (lane_id() == 0) {
+        if (nvalid) atomicAdd(&a.st->validSamples, nvalid);
+        if (noor) atomicAdd(&a.st->keyOutOfRange, noor);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= nb; k += blockDim.x) a.histR[(size_t)k * a.nblkR + blockIdx.x] = hist[k];
+}
+
+// --------------------------------------------------------------------------
+// Device-wide exclusive scan of uint32 counts (thrust::exclusive_scan /
+// the offsets of a stable counting sort).  Totals are kept in 64 bits.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* tot) {
+    __shared__ uint64_t wsum[SCAN_BLOCK / 64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t y = __shfl_up(x, o, 64);
+        if (l >= o) x += y;
+    }
+    if (l == 63) wsum[w] = x;
+    __syncthreads();
+    uint64_t off = 0, all = 0;
+    for (int i = 0; i < SCAN_BLOCK / 64; ++i) {
+        if (i < w) off += wsum[i];
+        all += wsum[i];
+    }
+    __syncthreads();
+    *tot = all;
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ d, long n,
+                                                            uint64_t* __restrict__ part, const int* status) {
+    if (status && *status != FD_EXIT_OK) return;
+    const long base = (long)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (base + k < n) sum += d[base + k];
+    uint64_t tot;
+    block_excl_scan_u64(sum, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_partials(uint64_t* __restrict__ part, int np,
+                                                              unsigned long long* total, const int* status) {
+    if (status && *status != FD_EXIT_OK) return;
+    uint64_t carry = 0;
+    for (int b0 = 0; b0 < np; b0 += SCAN_BLOCK) {
+        const int i = b0 + threadIdx.x;
+        const uint64_t v = i < np ? part[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan_u64(v, &tot);
+        if (i < np) part[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict__ d, long n,
+                                                           const uint64_t* __restrict__ part, const int* status) {
+    if (status && *status != FD_EXIT_OK) return;
+    const long base = (long)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (base + k < n) ? d[base + k] : 0u;
+        sum += v[k];
+    }
+    uint64_t tot;
+    uint64_t off = block_excl_scan_u64(sum, &tot) + part[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) d[base + k] = (uint32_t)off;
+        off += v[k];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Stable receiver scatter: one wave per block walks its RXB receivers in
+// order, 64 at a time; lanes holding equal keys are matched with ballots
+// over the key bits and ranked with mbcnt, so the result equals a stable
+// sort_by_key (Step3_bindSampleToBin.inl:125-127).  Binned receivers also
+// get their 32-byte occlusion record written in sorted order.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) {
+    extern __shared__ int cnt[];  // nb + 1, -1 = not loaded yet
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb;
+    int kbits = 1;
+    while ((1 << kbits) <= nb) ++kbits;
+    for (int k = threadIdx.x; k <= nb; k += 64) cnt[k] = -1;
+    __syncthreads();
+    const int base = blockIdx.x * FD_RXB;
+    for (int it = 0; it < FD_RXB / 64; ++it) {
+        const int i = base + it * 64 + lane_id();
+        const bool act = i < a.WH;
+        const int key = act ? (int)a.rxKey[i] : 0;
+        uint64_t eq = __ballot(act);
+        for (int b = 0; b < kbits; ++b) {
+            const bool bit = (key >> b) & 1;
+            const uint64_t bal = __ballot(act && bit);
+            eq &= bit ? bal : ~bal;
+        }
+        if (act) {
+            const int rank = mbcnt(eq);
+            const int gcount = __popcll(eq);
+            int b0 = cnt[key];
+            if (b0 < 0) b0 = (int)a.histR[(size_t)key * a.nblkR + blockIdx.x];
+            const int p = b0 + rank;
+            if (rank == gcount - 1) cnt[key] = b0 + gcount;
+            if ((unsigned)p < (unsigned)a.WH) a.sortedId[p] = i;
+            if (key < nb && (unsigned)p < (unsigned)a.WH) {
+                const float4 r = a.rec[i];
+                const RxRect q = rx_rect(r, s, a.L);
+                const float4 ps = a.pos[i];
+                FdRxRec o;
+                o.q[0] = ps.x; o.q[1] = ps.y; o.q[2] = ps.z;
+                o.rminx = q.bx; o.rminy = q.by; o.rmaxx = q.ex; o.rmaxy = q.ey; o.z = q.z;
+                a.sortedRec[p] = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// Triangle setup (triangleVertexSetup_kernel, Step0_triangleSetup.inl:56-191)
+// fused with the pair count (countPairNumForPerTri_kernel,
+// Step4_RasterizeTriangleToBin.inl:5-113).  The bin loop walks the union of
+// the wave's bin rectangles with a wave-uniform bin index, so the bin
+// statistics are scalar loads and the block histogram is updated once per
+// (wave, bin).
+// --------------------------------------------------------------------------
+struct BinStat { float maxZ, mx, my, mz, mw; };
+
+__device__ __forceinline__ BinStat load_bin(const uint32_t* bs) {
+    BinStat b;
+    b.maxZ = fdec(bs[0]); b.mx = fdec(bs[1]); b.my = fdec(bs[2]); b.mz = fdec(bs[3]); b.mw = fdec(bs[4]);
+    return b;
+}
+
+// the bin test of the count pass (:94), on the LP-clamped AABB
+__device__ __forceinline__ bool bin_test(const BinStat& b, float cbx, float cby, float cbz, float cex, float cey) {
+    return b.maxZ >= cbz && cbx < b.mx && cby < b.my && b.mz < cex && b.mw < cey;
+}
+
+struct TriClamp { float bx, by, bz, ex, ey; int sx, sy, ex_, ey_; bool ok; };
+
+// the count pass's triangle rejection, LP clamp and bin range (:41-83)
+__device__ __forceinline__ TriClamp tri_clamp(const float4 mn, const float4 mx, const FdState* s) {
+    TriClamp c;
+    const float B0 = s->B[0], B1 = s->B[1], E0 = s->E[0], E1 = s->E[1];
+    c.ok = !(mn.x > mx.x || mn.y > mx.y || mn.z > mx.z || mn.x > E0 || mn.y > E1 || mx.x < B0 || mx.y < B1);
+    c.bx = fmax_(mn.x, B0);
+    c.by = fmax_(mn.y, B1);
+    c.bz = mn.z;
+    c.ex = fmin_(mx.x, E0);
+    c.ey = fmin_(mx.y, E1);
+    const float G = (float)s->G;
+    c.sx = cvt_sat_u16(fmax_(floorf((c.bx - B0) * s->factor[0]), 0.0f));
+    c.sy = cvt_sat_u16(fmax_(floorf((c.by - B1) * s->factor[1]), 0.0f));
+    c.ex_ = cvt_sat_u16(fmin_(ceilf((c.ex - B0) * s->factor[0]), G));
+    c.ey_ = cvt_sat_u16(fmin_(ceilf((c.ey - B1) * s->factor[1]), G));
+    if (!c.ok) { c.sx = c.sy = 0; c.ex_ = c.ey_ = 0; }
+    return c;
+}
+
+__global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) {
+    extern __shared__ uint32_t thist[];  // nb
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb, log2G = s->log2G;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) thist[k] = 0;
+    __syncthreads();
+    const int t = blockIdx.x * FD_TRB + threadIdx.x;
+    const bool act = t < a.ntris;
+    const float* m = a.L.m;
+    float4 mn = make_float4(FD_F32_MAX, FD_F32_MAX, FD_F32_MAX, 1.0f);
+    float4 mx = make_float4(-FD_F32_MAX, -FD_F32_MAX, -FD_F32_MAX, 1.0f);
+    int valid = 0;
+    if (act) {
+        int i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
+        // an index outside the vertex buffer makes the triangle unusable
+        // (the reference reads out of bounds); it is treated as culled
+        const bool badIdx = (unsigned)i0 >= (unsigned)a.nverts || (unsigned)i1 >= (unsigned)a.nverts ||
+                            (unsigned)i2 >= (unsigned)a.nverts;
+        if (badIdx) i0 = i1 = i2 = 0;
+        const float* p0 = a.verts + (size_t)i0 * a.vstride;
+        const float* p1 = a.verts + (size_t)i1 * a.vstride;
+        const float* p2 = a.verts + (size_t)i2 * a.vstride;
+        const float v0[3] = {p0[0], p0[1], p0[2]}, v1[3] = {p1[0], p1[1], p1[2]}, v2[3] = {p2[0], p2[1], p2[2]};
+        // n = cross(v0 - v2, v1 - v0)   (:96-98)
+        const float e0x = v1[0] - v0[0], e0y = v1[1] - v0[1], e0z = v1[2] - v0[2];
+        const float e1x = v0[0] - v2[0], e1y = v0[1] - v2[1], e1z = v0[2] - v2[2];
+        const float nx = e1y * e0z - e1z * e0y, ny = e1z * e0x - e1x * e0z, nz = e1x * e0y - e1y * e0x;
+        bool culled = badIdx;
+        if (a.cull && !badIdx) {
+            const float* c[4] = {a.L.ur, a.L.ul, a.L.dr, a.L.dl};
+            const float* vv[3] = {v0, v1, v2};
+            culled = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    culled = culled && (dot3(nx, ny, nz, c[k][0] - vv[j][0], c[k][1] - vv[j][1], c[k][2] - vv[j][2]) <= 0);
+        }
+        if (!culled) {
+            const float* vv[3] = {v0, v1, v2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {  // getTriAABB (:37-54)
+                float rx, ry, rz;
+                makeRect(m, vv[k][0], vv[k][1], vv[k][2], rx, ry, rz);
+                mn.x = fmin_(mn.x, rx);
+                mn.y = fmin_(mn.y, ry);
+                mn.z = fmin_(mn.z, rz);
+                mx.z = fmax_(mx.z, rz);
+                mx.x = fmax_(mx.x, rx + a.L.F[0] / rz - s->S[0]);
+                mx.y = fmax_(mx.y, ry + a.L.F[1] / rz - s->S[1]);
+            }
+            if (!(mn.x > s->E[0] || mn.y > s->E[1] || mx.x < s->B[0] || mx.y < s->B[1])) {
+                valid = 1;
+                FdTri d;
+                for (int k = 0; k < 3; ++k) { d.v0[k] = v0[k]; d.v1[k] = v1[k]; d.v2[k] = v2[k]; }
+                d.bminx = mn.x; d.bminy = mn.y; d.bmaxx = mx.x; d.bmaxy = mx.y;
+                d.minz = mn.z; d.maxz = mx.z; d.pad0 = 0.0f;
+                // occlusion-side triangle constants, same expressions as
+                // Step5_2_shadowCalPerSample.inl:130 and :7
+                const float ax = v1[0] - v0[0], ay = v1[1] - v0[1], az = v1[2] - v0[2];
+                const float bx = v2[0] - v1[0], by = v2[1] - v1[1], bz = v2[2] - v1[2];
+                d.n[0] = ay * bz - az * by; d.n[1] = az * bx - ax * bz; d.n[2] = ax * by - ay * bx;
+                for (int k = 0; k < 3; ++k) {
+                    d.e01[k] = v0[k] - v1[k];
+                    d.e12[k] = v1[k] - v2[k];
+                    d.e20[k] = v2[k] - v0[k];
+                }
+                d.pad1[0] = d.pad1[1] = d.pad1[2] = d.pad1[3] = 0.0f;
+                a.tri[t] = d;
+            }
+        }
+        a.triAABB[2 * t] = mn;
+        a.triAABB[2 * t + 1] = mx;
+    }
+    {
+        const int nv = wave_sum_i32(valid);
+        if (lane_id() == 0 && nv) atomicAdd(&a.st->validTris, nv);
+    }
+    // count pass
+    TriClamp c = tri_clamp(mn, mx, s);
+    if (!act) { c.ok = false; c.sx = c.sy = c.ex_ = c.ey_ = 0; }
+    int count = 0, tsx = 6000, tsy = 6000, tex = 0, tey = 0;
+    const bool has = c.ok && c.sx < c.ex_ && c.sy < c.ey_;
+    const int ux0 = uniform(wave_min_i32(has ? c.sx : 1 << 20));
+    const int uy0 = uniform(wave_min_i32(has ? c.sy : 1 << 20));
+    const int ux1 = uniform(wave_max_i32(has ? c.ex_ : 0));
+    const int uy1 = uniform(wave_max_i32(has ? c.ey_ : 0));
+    for (int j = uy0; j < uy1; ++j) {
+        for (int i = ux0; i < ux1; ++i) {
+            const int bp = j << log2G | i;
+            const BinStat b = load_bin(a.binStat + 5 * bp);
+            const bool pass = has && j >= c.sy && j < c.ey_ && i >= c.sx && i < c.ex_ &&
+                              bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
+            const uint64_t bal = __ballot(pass);
+            if (pass) {
+                ++count;
+                tsx = min(tsx, i); tsy = min(tsy, j);
+                tex = max(tex, i + 1); tey = max(tey, j + 1);
+            }
+            if (bal && lane_id() == 0) atomicAdd(&thist[bp], (uint32_t)__popcll(bal));
+        }
+    }
+    if (act) {
+        a.triCount[t] = count;
+        a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blockIdx.x] = thist[k];
+}
+
+// Stable pair scatter (writeTriBinPair_kernel + sort_by_key,
+// Step4_RasterizeTriangleToBin.inl:161-223): per bin, triangles ascending.
+__global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) {
+    extern __shared__ int tcnt[];  // nb, -1 = not loaded yet
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb, log2G = s->log2G;
+    for (int k = threadIdx.x; k < nb; k += 64) tcnt[k] = -1;
+    __syncthreads();
+    const unsigned long long cap = a.pairCap;
+    for (int it = 0; it < FD_TRB / 64; ++it) {
+        const int t = blockIdx.x * FD_TRB + it * 64 + lane_id();
+        const bool act = t < a.ntris;
+        float4 mn = make_float4(0, 0, 0, 0), mx = mn;
+        uint2 rg = make_uint2(6000u | (6000u << 16), 0u);
+        if (act) { mn = a.triAABB[2 * t]; mx = a.triAABB[2 * t + 1]; rg = a.triRange[t]; }
+        const TriClamp c = tri_clamp(mn, mx, s);
+        const int sx = rg.x & 0xffff, sy = rg.x >> 16, ex = rg.y & 0xffff, ey = rg.y >> 16;
+        const bool has = act && c.ok && sx < ex && sy < ey;
+        const int ux0 = uniform(wave_min_i32(has ? sx : 1 << 20));
+        const int uy0 = uniform(wave_min_i32(has ? sy : 1 << 20));
+        const int ux1 = uniform(wave_max_i32(has ? ex : 0));
+        const int uy1 = uniform(wave_max_i32(has ? ey : 0));
+        for (int j = uy0; j < uy1; ++j) {
+            for (int i = ux0; i < ux1; ++i) {
+                const int bp = j << log2G | i;
+                const BinStat b = load_bin(a.binStat + 5 * bp);
+                const bool pass = has && j >= sy && j < ey && i >= sx && i < ex &&
+                                  bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
+                const uint64_t bal = __ballot(pass);
+                if (bal) {
+                    int b0 = tcnt[bp];
+                    if (b0 < 0) b0 = (int)a.histT[(size_t)bp * a.nblkT + blockIdx.x];
+                    if (pass) {
+                        const unsigned long long p = (unsigned long long)(uint32_t)b0 + (unsigned)mbcnt(bal);
+                        if (p < cap) a.sortedTri[p] = t;
+                    }
+                    if (lane_id() == 0) tcnt[bp] = b0 + __popcll(bal);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// Per-bin tables (FindStart_kernel, binRaster.inl:5-59) and the occlusion
+// work list: one work item per 64 sorted receivers of a bin (the invalid key
+// included, so every receiver's output is written exactly once).
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_tables(FdArgs a) {
+    __shared__ int wsum[16];
+    FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb;
+    int carry = 0;
+    for (int k0 = 0; k0 <= nb; k0 += 1024) {
+        const int k = k0 + threadIdx.x;
+        int nch = 0;
+        if (k <= nb) {
+            const int rs = (int)a.histR[(size_t)k * a.nblkR];
+            const int re = (k < nb) ? (int)a.histR[(size_t)(k + 1) * a.nblkR] : a.WH;
+            a.rxStart[k] = rs;
+            a.rxEnd[k] = re;
+            nch = (re - rs + 63) >> 6;
+            if (k < nb) {
+                const int ts = (int)a.histT[(size_t)k * a.nblkT];
+                const int te = (k < nb - 1) ? (int)a.histT[(size_t)(k + 1) * a.nblkT] : (int)s->pairs;
+                a.triStart[k] = ts;
+                a.triEnd[k] = te;
+            }
+        }
+        // block exclusive scan of nch
+        int x = nch;
+        const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o, 64);
+            if (l >= o) x += y;
+        }
+        if (l == 63) wsum[w] = x;
+        __syncthreads();
+        int off = carry, all = 0;
+        for (int i = 0; i < 16; ++i) {
+            if (i < w) off += wsum[i];
+            all += wsum[i];
+        }
+        if (k <= nb) a.chunkStart[k] = off + x - nch;
+        carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.chunkStart[nb + 1] = carry;
+        s->totalChunks = carry;
+    }
+}
+
+// Exit bookkeeping after the scans (setupTriangleVertex returns 1 when no
+// triangle reached the light plane, Step0_triangleSetup.inl:224-225;
+// FDLaunch skips the rest when there are no pairs, FD/api.cpp:522).
+__global__ void k_post_pairs(FdArgs a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    if (s->validTris == 0) s->status = FD_EXIT_NO_VALID_TRI;
+    else if (s->pairs == 0) s->status = FD_EXIT_NO_PAIRS;
+    else if (s->pairs > a.pairCap || s->pairs >= 0x7fffffffull) { s->status = FD_EXIT_CAPACITY; s->capacityHit = 1; }
+}
+
+// --------------------------------------------------------------------------
+// Occlusion (shadowCal_per_sample_kernel, Step5_2_shadowCalPerSample.inl:37-234).
+// One wave per work item: 64 sorted receivers of one bin, one receiver per
+// lane.  The bin's triangle list is walked with a wave-uniform index, so
+// triangle records arrive through scalar loads; each lane keeps its light
+// rows in LDS (row-major by row, lane-minor: conflict-free).
+// --------------------------------------------------------------------------
+template <typename T>
+struct Bits;
+template <>
+struct Bits<uint32_t> {
+    static constexpr int width = 32;
+    __device__ static int ctz(uint32_t v) { return __builtin_ctz(v); }
+    __device__ static int popc(uint32_t v) { return __popc(v); }
+};
+template <>
+struct Bits<uint64_t> {
+    static constexpr int width = 64;
+    __device__ static int ctz(uint64_t v) { return __builtin_ctzll(v); }
+    __device__ static int popc(uint64_t v) { return __popcll(v); }
+};
+
+// getRowMaskForPlaneDiscriminant (:14-33): the mask of light samples along
+// one row (or of rows) whose plane value is non-negative.  Bn = B * n is
+// passed precomputed (the reference evaluates B * n + D).  Shifts outside
+// [0, width) give 0, conversions saturate; the caller clips to the grid.
+template <typename T>
+__device__ __forceinline__ T row_mask(float D, float B, float Bn) {
+    const bool f1 = (D >= 0.0f);
+    const bool f2 = (Bn + D >= 0.0f);
+    if (f1 == f2) return f1 ? ~(T)0 : (T)0;
+    const int k = cvt_sat(ceilf(-1.0f * D / B));
+    T m = (k >= 0 && k < Bits<T>::width) ? (((T)1 << k) - (T)1) : ~(T)0;
+    return f1 ? m : ~m;
+}
+
+// test3DPointInTriangle, Step5_2_shadowCal.inl:213-222
+__device__ __forceinline__ bool point_in_tri(const float* A, const float* B, const float* C, float px, float py, float pz) {
+    auto cross = [](float ax, float ay, float az, float bx, float by, float bz, float* o) {
+        o[0] = ay * bz - az * by; o[1] = az * bx - ax * bz; o[2] = ax * by - ay * bx;
+    };
+    float N1[3], N2[3], N3[3];
+    cross(B[0] - A[0], B[1] - A[1], B[2] - A[2], px - B[0], py - B[1], pz - B[2], N1);
+    cross(C[0] - B[0], C[1] - B[1], C[2] - B[2], px - C[0], py - C[1], pz - C[2], N2);
+    if (dot3(N1[0], N1[1], N1[2], N2[0], N2[1], N2[2]) < 0.0f) return false;
+    cross(A[0] - C[0], A[1] - C[1], A[2] - C[2], px - A[0], py - A[1], pz - A[2], N3);
+    if (dot3(N2[0], N2[1], N2[2], N3[0], N3[1], N3[2]) < 0.0f) return false;
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, const int* ids, int n, int R, int wpr) {
+    // coalesced write-out: consecutive lanes write consecutive words of one
+    // receiver's mask (R rows x wpr words)
+    const int per = R * wpr;
+    const int total = n * per;
+    for (int j = lane_id(); j < total; j += 64) {
+        const int l = j / per, w = j - l * per;
+        const int r = w / wpr, h = w - r * wpr;
+        a.masks[(size_t)ids[l] * per + w] = lds[(h * R + r) * 64 + l];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
+    extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 ids
+    const FdState* s = a.st;
+    const int status = s->status;
+    const int R = a.L.res;                  // rows == columns
+    const int wpr = a.L.wpr;
+    const int lane = lane_id();
+    int* ids = (int*)(lds + wpr * R * 64);
+    const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
+    const int c = blockIdx.x;
+
+    if (status != FD_EXIT_OK) {
+        // early exit of the frame: every receiver shadow 0, except back-facing
+        // receivers once the back-face stage has run (FD/api.cpp:489,
+        // Step2_1_samplerRectangleAABB.inl:97)
+        const int i0 = c * 64;
+        if (i0 >= a.WH) return;
+        const int n = min(64, a.WH - i0);
+        const bool backStage = status >= FD_EXIT_NO_VALID_SAMPLE;
+        bool back = false;
+        if (lane < n) {
+            back = backStage && __float_as_uint(a.rec[i0 + lane].w) == FD_TAG_BACK;
+            a.shadow[i0 + lane] = back ? 1.0f : 0.0f;
+            ids[lane] = i0 + lane;
+        }
+        for (int r = 0; r < R; ++r) {
+            const T v = back ? FULL : (T)0;
+            lds[r * 64 + lane] = (uint32_t)v;
+            if (wpr == 2) lds[(R + r) * 64 + lane] = (uint32_t)((uint64_t)v >> 32);
+        }
+        __syncthreads();
+        store_masks<T>(a, lds, ids, n, R, wpr);
+        return;
+    }
+    if (c >= s->totalChunks) return;
+    const int nb = s->nb;
+    // work item -> bin (binary search over the chunk prefix)
+    int lo = 0, hi = nb + 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.chunkStart[mid] <= c) lo = mid; else hi = mid;
+    }
+    const int bin = uniform(lo);
+    const int s0 = a.rxStart[bin] + (c - a.chunkStart[bin]) * 64;
+    const int n = min(64, a.rxEnd[bin] - s0);
+    const bool act = lane < n;
+    int id = 0;
+    if (act) id = a.sortedId[s0 + lane];
+    if ((unsigned)id >= (unsigned)a.WH) id = 0;   // defensive: ids form a permutation
+    ids[lane] = id;
+
+    if (bin == nb) {
+        // receivers outside the light plane, invalid or back-facing
+        bool back = false;
+        if (act) {
+            back = __float_as_uint(a.rec[id].w) == FD_TAG_BACK;
+            a.shadow[id] = back ? 1.0f : 0.0f;
+        }
+        for (int r = 0; r < R; ++r) {
+            const T v = back ? FULL : (T)0;
+            lds[r * 64 + lane] = (uint32_t)v;
+            if (wpr == 2) lds[(R + r) * 64 + lane] = (uint32_t)((uint64_t)v >> 32);
+        }
+        __syncthreads();
+        store_masks<T>(a, lds, ids, n, R, wpr);
+        return;
+    }
+
+    FdRxRec q{};
+    if (act) q = a.sortedRec[s0 + lane];
+    for (int r = 0; r < R; ++r) {
+        lds[r * 64 + lane] = 0u;
+        if (wpr == 2) lds[(R + r) * 64 + lane] = 0u;
+    }
+    const float Lx = a.L.ur[0] - q.q[0], Ly = a.L.ur[1] - q.q[1], Lz = a.L.ur[2] - q.q[2];  // L0 - Q
+    const float xd0 = a.L.xd[0], xd1 = a.L.xd[1], xd2 = a.L.xd[2];
+    const float yd0 = a.L.yd[0], yd1 = a.L.yd[1], yd2 = a.L.yd[2];
+    const float nsub = (float)(unsigned)(R - 1);
+    const float eps = a.eps;
+    T full = 0;
+    const int ts = a.triStart[bin], te = a.triEnd[bin];
+    // read-only for this kernel: the constant address space lets the uniform
+    // per-triangle loads go through the scalar unit (s_load)
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(4))) int* cint_p;
+    typedef const __attribute__((address_space(4))) FdTri* ctri_p;
+#else
+    typedef const int* cint_p;   // host pass: the kernel body is never run
+    typedef const FdTri* ctri_p;
+#endif
+    const cint_p triList = (cint_p)(a.sortedTri);
+    const ctri_p tris = (ctri_p)(a.tri);
+    for (int ti = ts; ti < te; ++ti) {
+        const bool live = act && full != FULL;
+        if (__ballot(live) == 0) break;
+        const int tid = triList[ti];
+        if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
+        const FdTri t = tris[tid];
+        if (!live) continue;
+        // containment + depth cull (:120-127)
+        if (!(t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx && q.rmaxy <= t.bmaxy && t.minz <= q.z))
+            continue;
+        // dir = dot(Q - v0, n) (:130-132)
+        const float dx = q.q[0] - t.v0[0], dy = q.q[1] - t.v0[1], dz = q.q[2] - t.v0[2];
+        const float dir = dot3(dx, dy, dz, t.n[0], t.n[1], t.n[2]);
+        if (fabsf(dir) < eps && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+        // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172).  The reference
+        // negates t0, t1, L0_Q when dir < 0; negation commutes with every
+        // rounding here, so A, B, C are negated instead (identical values up
+        // to the sign of zero results, which no later step can observe).
+        float A[3], Bv[3], C[3];
+        const float* vb[3] = {t.v1, t.v2, t.v0};
+        const float* ev[3] = {t.e01, t.e12, t.e20};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float ax = vb[k][0] - q.q[0], ay = vb[k][1] - q.q[1], az = vb[k][2] - q.q[2];
+            const float bx = ev[k][0], by = ev[k][1], bz = ev[k][2];
+            const float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+            A[k] = dot3(nx, ny, nz, xd0, xd1, xd2);
+            Bv[k] = dot3(nx, ny, nz, yd0, yd1, yd2);
+            C[k] = dot3(nx, ny, nz, Lx, Ly, Lz);
+            if (dir < 0) { A[k] = -A[k]; Bv[k] = -Bv[k]; C[k] = -C[k]; }
+        }
+        // rows that may hold occluded samples (:176-191)
+        float Bn[3];
+        T N = FULL;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Bn[k] = Bv[k] * nsub;
+            const float D = (Bv[k] > 0 ? Bn[k] : 0.0f) + C[k];
+            N &= row_mask<T>(D, A[k], A[k] * nsub);
+        }
+        T valid = (~full) & N;
+        // row by row, lowest row first (:204-222)
+        while (valid) {
+            const int r = Bits<T>::ctz(valid);
+            const float fr = (float)r;
+            T M = FULL;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) M &= row_mask<T>(A[k] * fr + C[k], Bv[k], Bn[k]);
+            uint32_t* w0 = lds + r * 64 + lane;
+            T cur;
+            if (Bits<T>::width == 32) {
+                cur = (T)(*w0 | (uint32_t)M);
+                *w0 = (uint32_t)cur;
+            } else {
+                uint32_t* w1 = lds + (R + r) * 64 + lane;
+                const uint64_t mm = (uint64_t)M;
+                const uint32_t lo32 = *w0 | (uint32_t)mm, hi32 = *w1 | (uint32_t)(mm >> 32);
+                *w0 = lo32;
+                *w1 = hi32;
+                cur = (T)(((uint64_t)hi32 << 32) | lo32);
+            }
+            if (cur == FULL) full |= (T)1 << r;
+            valid &= valid - (T)1;
+        }
+    }
+    // fraction = popcount / (W*H) (:226-231)
+    int cnt = 0;
+    for (int r = 0; r < R; ++r) {
+        cnt += __popc(lds[r * 64 + lane]);
+        if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
+    }
+    if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
+    __syncthreads();
+    store_masks<T>(a, lds, ids, n, R, wpr);
+}
+
+// Stage read-back of the receiver rectangles in the reference's 2-layer
+// layout (Step2_1_samplerRectangleAABB.inl:122-123); verification only.
+__global__ void k_debug_rect(FdArgs a, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.WH) return;
+    const float4 r = a.rec[i];
+    float v[8] = {FD_F32_MAX, FD_F32_MAX, FD_F32_MAX, 1.0f, -FD_F32_MAX, -FD_F32_MAX, -FD_F32_MAX, 1.0f};
+    if (__float_as_uint(r.w) == FD_TAG_VALID) {
+        const RxRect q = rx_rect(r, a.st, a.L);
+        v[0] = q.bx; v[1] = q.by; v[2] = q.z; v[4] = q.ex; v[5] = q.ey; v[6] = q.z;
+    }
+    for (int k = 0; k < 8; ++k) out[(size_t)i * 8 + k] = v[k];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host-side launch sequence of one frame.
+// ===========================================================================
+namespace fdk {
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+static hipError_t scan_u32(uint32_t* d, long n, uint64_t* part, unsigned long long* total, const int* status,
+                           hipStream_t st) {
+    const int nch = cdiv(n, SCAN_CHUNK);
+    k_scan_reduce<<<nch, SCAN_BLOCK, 0, st>>>(d, n, part, status);
+    k_scan_partials<<<1, SCAN_BLOCK, 0, st>>>(part, nch, total, status);
+    k_scan_apply<<<nch, SCAN_BLOCK, 0, st>>>(d, n, part, status);
+    return hipGetLastError();
+}
+
+long scan_partials_needed(long n) { return cdiv(n, SCAN_CHUNK); }
+
+hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st) {
+    k_debug_rect<<<cdiv(a.WH, 256), 256, 0, st>>>(a, out8);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
+hipError_t enqueue_frame(const FdArgs& a, hipStream_t st) {
+    const int nb = 1 << (2 * a.gridLog2);
+    const int nbR = cdiv(a.WH, RED_BLOCK * RED_ITEMS);
+    const int nbV = cdiv(a.nverts, RED_BLOCK * RED_ITEMS);
+    FdState* s = a.st;
+    k_init<<<1, 64, 0, st>>>(a);
+    k_init_bins<<<cdiv(nb, 256), 256, 0, st>>>(a);
+    k_reduce<<<nbR + nbV, RED_BLOCK, 0, st>>>(a, nbR);
+    k_finalize<<<1, 64, 0, st>>>(a);
+    // receivers: key + bin statistics, stable counting sort
+    k_rx_key<<<a.nblkR, 256, (nb + 1) * sizeof(uint32_t), st>>>(a);
+    hipError_t e = scan_u32(a.histR, (long)(nb + 1) * a.nblkR, a.scanPart, &s->rxTotal, &s->status, st);
+    if (e != hipSuccess) return e;
+    k_rx_scatter<<<a.nblkR, 64, (nb + 1) * sizeof(int), st>>>(a);
+    // triangles: setup + pair count, stable pair scatter
+    if (a.ntris > 0) {
+        k_tri_count<<<a.nblkT, FD_TRB, nb * sizeof(uint32_t), st>>>(a);
+        e = scan_u32(a.histT, (long)nb * a.nblkT, a.scanPart, &s->pairs, &s->status, st);
+        if (e != hipSuccess) return e;
+        k_post_pairs<<<1, 64, 0, st>>>(a);
+        k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);
+    } else {
+        k_post_pairs<<<1, 64, 0, st>>>(a);
+    }
+    k_tables<<<1, 1024, 0, st>>>(a);
+    // occlusion (or the early-exit fill)
+    const int chunks = cdiv(a.WH, 64) + nb + 1;
+    const int wpr = a.L.wpr;
+    const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * sizeof(int);
+    if (wpr == 1) k_occlude<uint32_t><<<chunks, 64, shm, st>>>(a);
+    else k_occlude<uint64_t><<<chunks, 64, shm, st>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace fdk

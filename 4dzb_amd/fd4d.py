@@ -1,0 +1,266 @@
+"""ctypes binding of libfd4d.so (include/fd4d.h).
+
+Device buffers are passed as raw device pointers; the helpers below take
+torch CUDA tensors for convenience (torch is only used as a device-memory
+and stream provider).  The library must be present: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfd4d.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "fd4d.h")
+
+FD_OK = 0
+FD_ERR_ARG = -1
+FD_ERR_STATE = -2
+FD_ERR_HIP = -3
+FD_ERR_CAPACITY = -4
+FD_ERR_UNSUPPORTED_LIGHT = -5
+FD_ERR_NO_DEVICE = -6
+
+EXIT_OK, EXIT_SAMPLE_MAXZ, EXIT_TRI_MAXZ, EXIT_NO_VALID_SAMPLE = 0, 1, 2, 3
+EXIT_EMPTY_AABB, EXIT_EMPTY_LP, EXIT_NO_VALID_TRI, EXIT_NO_PAIRS, EXIT_CAPACITY = 4, 5, 6, 7, 9
+
+
+class AreaLight(ctypes.Structure):
+    _fields_ = [("right_radius", ctypes.c_float), ("top_radius", ctypes.c_float),
+                ("position", ctypes.c_float * 3), ("view_dir", ctypes.c_float * 3), ("up_dir", ctypes.c_float * 3),
+                ("res_w", ctypes.c_int), ("res_h", ctypes.c_int)]
+
+    @classmethod
+    def from_desc(cls, d):
+        return cls(d.right_radius, d.top_radius, (ctypes.c_float * 3)(*d.position),
+                   (ctypes.c_float * 3)(*d.view_dir), (ctypes.c_float * 3)(*d.up_dir), d.res_w, d.res_h)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("exit_code", ctypes.c_int), ("valid_receivers", ctypes.c_int), ("front_receivers", ctypes.c_int),
+                ("valid_triangles", ctypes.c_int), ("pairs", ctypes.c_ulonglong), ("key_out_of_range", ctypes.c_int),
+                ("work_items", ctypes.c_int), ("elapsed_ms", ctypes.c_float),
+                ("light_plane_begin", ctypes.c_float * 2), ("light_plane_end", ctypes.c_float * 2),
+                ("far_plane", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if hasattr(getattr(self, k), "__len__") else getattr(self, k))
+                for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libfd4d.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+        sig = {
+            "fd_init": ([ctypes.POINTER(vp)], ip),
+            "fd_destroy": ([vp], ip),
+            "fd_set_backface_cull": ([vp, ip], ip),
+            "fd_add_model": ([vp, vp, sz, ip, vp, ip], ip),
+            "fd_clear_models": ([vp], ip),
+            "fd_set_sample_buffers": ([vp, vp, vp], ip),
+            "fd_set_output_buffers": ([vp, vp, vp], ip),
+            "fd_register_res": ([vp], ip),
+            "fd_unregister_res": ([vp], ip),
+            "fd_set_bin_num": ([vp, sz], ip),
+            "fd_set_view": ([vp, ip, ip], ip),
+            "fd_set_scenes_esp": ([vp, ctypes.c_float], ip),
+            "fd_set_scene_params": ([vp, sz, sz], ip),
+            "fd_set_area_light": ([vp, ctypes.POINTER(AreaLight)], ip),
+            "fd_launch": ([vp, vp, ctypes.POINTER(Stats)], ip),
+            "fd_launch_async": ([vp, vp], ip),
+            "fd_get_stats": ([vp, vp, ctypes.POINTER(Stats)], ip),
+            "fd_set_pair_capacity": ([vp, sz], ip),
+            "fd_move_model": ([vp, ctypes.POINTER(ctypes.c_float), vp], ip),
+            "fd_mask_words": ([ip], ip),
+            "fd_debug_read": ([vp, ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)], ip),
+            "fd_status_string": ([ip], ctypes.c_char_p),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def mask_words(res):
+    return lib().fd_mask_words(res)
+
+
+class FdError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != FD_OK:
+        raise FdError(f"{what}: {lib().fd_status_string(rc).decode()} ({rc})")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class Context:
+    """One fd_ctx.  Mirrors the FD:: call sequence of the reference."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().fd_init(ctypes.byref(h)), "fd_init")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().fd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_backface_cull(self, on):
+        _check(lib().fd_set_backface_cull(self.h, int(on)), "fd_set_backface_cull")
+
+    def set_scenes_esp(self, eps):
+        _check(lib().fd_set_scenes_esp(self.h, eps), "fd_set_scenes_esp")
+
+    def set_bin_num(self, n):
+        rc = lib().fd_set_bin_num(self.h, n)
+        if rc != FD_OK:
+            raise FdError(f"fd_set_bin_num({n}) -> {rc}")
+
+    def set_pair_capacity(self, n):
+        _check(lib().fd_set_pair_capacity(self.h, n), "fd_set_pair_capacity")
+
+    def add_model(self, verts_dev, idx_dev):
+        """verts_dev: (V, k>=3) float32 CUDA tensor; idx_dev: (T, 3) int32."""
+        lib().fd_clear_models(self.h)
+        stride = verts_dev.shape[1] * 4
+        _check(lib().fd_add_model(self.h, _ptr(verts_dev), stride, verts_dev.shape[0], _ptr(idx_dev),
+                                  idx_dev.shape[0]), "fd_add_model")
+        self._keep += [verts_dev, idx_dev]
+
+    def set_receivers(self, pos_dev, nrm_dev):
+        h, w = pos_dev.shape[0], pos_dev.shape[1]
+        _check(lib().fd_set_view(self.h, w, h), "fd_set_view")
+        _check(lib().fd_set_sample_buffers(self.h, _ptr(pos_dev), _ptr(nrm_dev)), "fd_set_sample_buffers")
+        self._keep += [pos_dev, nrm_dev]
+
+    def set_outputs(self, shadow_dev, masks_dev=None):
+        _check(lib().fd_set_output_buffers(self.h, _ptr(shadow_dev), _ptr(masks_dev)), "fd_set_output_buffers")
+        self._keep += [shadow_dev, masks_dev]
+
+    def set_area_light(self, desc):
+        L = AreaLight.from_desc(desc)
+        _check(lib().fd_set_area_light(self.h, ctypes.byref(L)), "fd_set_area_light")
+
+    def launch(self, stream=None):
+        st = Stats()
+        rc = lib().fd_launch(self.h, ctypes.c_void_p(stream or 0), ctypes.byref(st))
+        return rc, st
+
+    def launch_async(self, stream=None):
+        _check(lib().fd_launch_async(self.h, ctypes.c_void_p(stream or 0)), "fd_launch_async")
+
+    def get_stats(self, stream=None):
+        st = Stats()
+        rc = lib().fd_get_stats(self.h, ctypes.c_void_p(stream or 0), ctypes.byref(st))
+        return rc, st
+
+    def move_model(self, m16, stream=None):
+        arr = (ctypes.c_float * 16)(*[float(x) for x in m16])
+        _check(lib().fd_move_model(self.h, arr, ctypes.c_void_p(stream or 0)), "fd_move_model")
+
+    def debug_read(self, name, dtype):
+        n = ctypes.c_size_t()
+        _check(lib().fd_debug_read(self.h, name.encode(), None, 0, ctypes.byref(n)), f"fd_debug_read({name})")
+        out = np.empty(n.value // np.dtype(dtype).itemsize, dtype)
+        if n.value:
+            _check(lib().fd_debug_read(self.h, name.encode(), ctypes.c_void_p(out.ctypes.data), n.value,
+                                       ctypes.byref(n)), f"fd_debug_read({name})")
+        return out
+
+
+class FrameRunner:
+    """Uploads a workloads.Frame to the GPU and runs it through one Context."""
+
+    def __init__(self, frame, with_masks=True, ctx=None, pair_capacity=None):
+        import torch
+        self.torch = torch
+        self.frame = frame
+        dev = torch.device("cuda", 0)
+        self.verts = torch.from_numpy(frame.scene.verts).to(dev)
+        self.idx = torch.from_numpy(frame.scene.idx).to(dev)
+        self.pos = torch.from_numpy(frame.rx.pos).to(dev)
+        self.nrm = torch.from_numpy(frame.rx.nrm).to(dev)
+        W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+        self.shadow = torch.empty(H * W, dtype=torch.float32, device=dev)
+        self.masks = None
+        if with_masks and 1 <= R <= 64:
+            self.masks = torch.empty(H * W * mask_words(R), dtype=torch.int32, device=dev)
+        self.ctx = ctx or Context()
+        c = self.ctx
+        c.set_backface_cull(frame.backface_cull)
+        c.set_scenes_esp(frame.eps)
+        c.set_bin_num(1 << frame.grid_log2)
+        if pair_capacity:
+            c.set_pair_capacity(pair_capacity)
+        c.add_model(self.verts, self.idx)
+        c.set_receivers(self.pos, self.nrm)
+        c.set_outputs(self.shadow, self.masks)
+        self.light_rc = lib().fd_set_area_light(c.h, ctypes.byref(AreaLight.from_desc(frame.light)))
+
+    def run(self):
+        rc, st = self.ctx.launch(self.torch.cuda.current_stream().cuda_stream)
+        return rc, st
+
+    def outputs(self):
+        sh = self.shadow.cpu().numpy()
+        mk = self.masks.cpu().numpy().view(np.uint32) if self.masks is not None else None
+        return sh, mk
+
+
+# numpy mirrors of fd_types.h records (for stage read-back)
+STATE_DTYPE = np.dtype([
+    ("rxZmaxAll", "<u4"), ("rxMin", "<u4", 3), ("rxMaxA", "<u4", 2), ("rxMaxZ", "<u4"),
+    ("vxZmax", "<u4"), ("vxMin", "<u4", 3), ("vxMaxA", "<u4", 2), ("vxMaxZ", "<u4"),
+    ("validRect", "<i4"), ("status", "<i4"), ("validSamples", "<i4"), ("validTris", "<i4"),
+    ("keyOutOfRange", "<i4"), ("capacityHit", "<i4"), ("pairs", "<u8"), ("rxTotal", "<u8"),
+    ("totalChunks", "<i4"), ("pad0", "<i4"), ("sampleMaxZ", "<f4"), ("triMaxZ", "<f4"), ("f", "<f4"),
+    ("S", "<f4", 2), ("sAABB", "<f4", 8), ("mAABB", "<f4", 8), ("B", "<f4", 3), ("E", "<f4", 3),
+    ("factor", "<f4", 2), ("G", "<i4"), ("log2G", "<i4"), ("nb", "<i4"), ("pad1", "<i4")], align=True)
+
+LIGHT_DTYPE = np.dtype([("m", "<f4", 16), ("xd", "<f4", 3), ("yd", "<f4", 3), ("ur", "<f4", 3), ("ul", "<f4", 3),
+                        ("dl", "<f4", 3), ("dr", "<f4", 3), ("F", "<f4", 2), ("res", "<i4"), ("resNum", "<i4"),
+                        ("wpr", "<i4"), ("pad", "<i4")])
+
+TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), ("bminx", "<f4"), ("bminy", "<f4"),
+                      ("bmaxx", "<f4"), ("bmaxy", "<f4"), ("minz", "<f4"), ("maxz", "<f4"), ("pad0", "<f4"),
+                      ("n", "<f4", 3), ("e01", "<f4", 3), ("e12", "<f4", 3), ("e20", "<f4", 3), ("pad1", "<f4", 4)])
+
+
+def read_state(ctx):
+    raw = ctx.debug_read("state", np.uint8)
+    return np.frombuffer(raw.tobytes()[:STATE_DTYPE.itemsize], STATE_DTYPE)[0]
+
+
+def read_light(ctx):
+    raw = ctx.debug_read("light", np.uint8)
+    return np.frombuffer(raw.tobytes()[:LIGHT_DTYPE.itemsize], LIGHT_DTYPE)[0]
+
+
+def read_tris(ctx):
+    raw = ctx.debug_read("tri_data", np.uint8)
+    return np.frombuffer(raw.tobytes(), TRI_DTYPE)

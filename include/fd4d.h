@@ -1,0 +1,169 @@
+/*
+ * fd4d.h — C ABI of the MI355X soft-shadow visibility library (libfd4d.so).
+ *
+ * Drop-in for the FDZB_Soft_Shadow call surface of galaxy-wolf/4DZB
+ * (namespace FD, 4DZB/FDZB_Soft_Shadow/api.h:10-56).  Each entry point below
+ * names the reference function it replaces.  Differences that follow from
+ * the boundary, not from the algorithm:
+ *   - OpenGL handles become device pointers (the caller owns every buffer);
+ *   - the reference's process globals become an explicit context;
+ *   - errors are returned as status codes instead of terminating the process;
+ *   - outputs are dense (y * width + x); the reference writes the shadow value
+ *     at y << ceil(log2 width) | x, which only differs for non-power-of-two
+ *     widths;
+ *   - besides the reference's per-receiver shadow value (occluded fraction,
+ *     1 - visibility) the library also writes the per-receiver light-sample
+ *     bitmask (bit set = sample occluded).
+ *
+ * Mask layout: receiver i owns fd_mask_words(res) consecutive uint32 words:
+ * res rows of light samples, row r = r steps along the light's x_delt
+ * (rightDir * -2 * rightRadius / res), bit c = c steps along y_delt.  Grids
+ * up to 32 samples wide use one word per row; wider grids (up to 64) use two
+ * (low word first).  Back-facing receivers have every bit set and shadow 1.
+ *
+ * Threading: one context per host thread; calls are stream-ordered on the
+ * stream passed to the launch functions.
+ */
+#ifndef FD4D_H
+#define FD4D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fd_ctx fd_ctx;
+
+/* Status codes (return values). */
+#define FD_OK 0
+#define FD_ERR_ARG (-1)               /* invalid argument */
+#define FD_ERR_STATE (-2)             /* required set-up call missing */
+#define FD_ERR_HIP (-3)               /* HIP runtime error */
+#define FD_ERR_CAPACITY (-4)          /* pair list larger than the pair arena */
+#define FD_ERR_UNSUPPORTED_LIGHT (-5) /* light grid not square or outside 1..64 */
+#define FD_ERR_NO_DEVICE (-6)         /* no GPU / kernels not loadable */
+
+/* Frame exit codes (fd_stats.exit_code), in the order of FD::FDLaunch's
+ * early returns (4DZB/FDZB_Soft_Shadow/api.cpp:495-559). */
+#define FD_EXIT_OK 0              /* occlusion evaluated */
+#define FD_EXIT_SAMPLE_MAXZ 1     /* no receiver in front of the light */
+#define FD_EXIT_TRI_MAXZ 2        /* no vertex in front of the light */
+#define FD_EXIT_NO_VALID_SAMPLE 3 /* every receiver back-facing or empty */
+#define FD_EXIT_EMPTY_AABB 4
+#define FD_EXIT_EMPTY_LP 5        /* receiver and occluder projections disjoint */
+#define FD_EXIT_NO_VALID_TRI 6
+#define FD_EXIT_NO_PAIRS 7
+#define FD_EXIT_CAPACITY 9
+
+/* AreaLightDes, 4DZB/FDZB_Soft_Shadow/AreaLightDesc.h:3-13. */
+typedef struct fd_area_light {
+    float right_radius; /* half width */
+    float top_radius;   /* half height */
+    float position[3];
+    float view_dir[3];
+    float up_dir[3];
+    int res_w;          /* light samples per row; res_w == res_h, 1..64 */
+    int res_h;
+} fd_area_light;
+
+/* Per-frame record (the reference only returns the elapsed ms). */
+typedef struct fd_stats {
+    int exit_code;           /* FD_EXIT_* */
+    int valid_receivers;     /* receivers bound to a light-plane bin */
+    int front_receivers;     /* non-empty, front-facing receivers */
+    int valid_triangles;     /* triangles reaching the light plane */
+    unsigned long long pairs;/* (bin, triangle) pairs */
+    int key_out_of_range;    /* bound receivers whose bin key left the grid */
+    int work_items;          /* occlusion work items (64 receivers each) */
+    float elapsed_ms;        /* device time of the frame (fd_launch only) */
+    float light_plane_begin[2];
+    float light_plane_end[2];
+    float far_plane;         /* f = max(receiver z, vertex z) + 0.1 */
+} fd_stats;
+
+/* FD::FDinit, FD/api.cpp:48-144.  Creates a context on the current device. */
+int fd_init(fd_ctx** out);
+/* No counterpart (the reference keeps process globals): frees the context. */
+int fd_destroy(fd_ctx* ctx);
+
+/* FD::FDsetBackfaceCull, FD/api.cpp:146-150 (default on, FourD.cu:61). */
+int fd_set_backface_cull(fd_ctx* ctx, int enable);
+
+/* FD::FDaddModel, FD/api.cpp:152-160.  Vertex positions are the first three
+ * floats of each vertex; vertex_size_bytes is the stride (multiple of 4,
+ * >= 12).  Indices are int3 per triangle.  Device pointers, caller-owned.
+ * As in the reference only the first model added is used
+ * (FD/api.cpp:430,441-445); fd_clear_models drops them all. */
+int fd_add_model(fd_ctx* ctx, const float* d_vertices, size_t vertex_size_bytes, int num_vertices,
+                 const int* d_indices, int num_triangles);
+int fd_clear_models(fd_ctx* ctx);
+
+/* FD::FDsetSampleGLTex, FD/api.cpp:162-166.  Receiver G-buffer: float4
+ * positions and normals, width*height each, row-major; pos.x NaN = empty. */
+int fd_set_sample_buffers(fd_ctx* ctx, const float* d_position4, const float* d_normal4);
+
+/* FD::FDsetOutputGLBuffer, FD/api.cpp:168-171.  d_shadow: width*height
+ * floats.  d_masks: width*height*fd_mask_words(res) words, or NULL. */
+int fd_set_output_buffers(fd_ctx* ctx, float* d_shadow, uint32_t* d_masks);
+
+/* FD::FDregisterGLres / FD::FDunregisterGLres, FD/api.cpp:173-192.  There is
+ * no interop to register: these validate the bound buffers / drop them. */
+int fd_register_res(fd_ctx* ctx);
+int fd_unregister_res(fd_ctx* ctx);
+
+/* FD::FDsetBinNum, declared at FD/api.h:36 (body commented out,
+ * FD/api.cpp:193-202): bins per light-plane side, a power of two in 8..64
+ * (default 16 = FD_MAX_LP_GRID_SIZE, constants.h:13,45). */
+int fd_set_bin_num(fd_ctx* ctx, size_t bins_per_side);
+
+/* FD::FDsetView, FD/api.cpp:205-237. */
+int fd_set_view(fd_ctx* ctx, int width, int height);
+
+/* FD::FDsetScenesEsp, FD/api.cpp:238-241 (default 1e-6, FourD.cu:62). */
+int fd_set_scenes_esp(fd_ctx* ctx, float eps);
+
+/* FD::FDsetSceneParams, FD/api.cpp:243-294: capacity for triangles/vertices. */
+int fd_set_scene_params(fd_ctx* ctx, size_t max_triangles, size_t max_vertices);
+
+/* FD::FDsetAreaLight, FD/api.cpp:303-400.  Unlike the reference, launches
+ * stay correct without calling this again before every frame. */
+int fd_set_area_light(fd_ctx* ctx, const fd_area_light* light);
+
+/* FD::FDLaunch, FD/api.cpp:402-582.  Runs one frame on `hip_stream` (NULL =
+ * default stream), waits for it and fills *stats (may be NULL).  Returns
+ * FD_OK for every exit code except FD_EXIT_CAPACITY (FD_ERR_CAPACITY). */
+int fd_launch(fd_ctx* ctx, void* hip_stream, fd_stats* stats);
+
+/* No counterpart: enqueue one frame without any host synchronisation (the
+ * sequence is capturable into a HIP graph); fd_get_stats reads the result. */
+int fd_launch_async(fd_ctx* ctx, void* hip_stream);
+int fd_get_stats(fd_ctx* ctx, void* hip_stream, fd_stats* stats);
+
+/* No counterpart (the reference's pair buffer is fixed at
+ * FD_MAX_BT_PIAR_Num * 1.4 entries, FD/api.cpp:132-138): pair arena size. */
+int fd_set_pair_capacity(fd_ctx* ctx, size_t max_pairs);
+
+/* FD::moveModel, FD/api.cpp:17-20 / cudaModeTransform, FDc/moveModel.inl:16-51:
+ * in-place 4x4 (column-major) transform of model 0's vertex positions. */
+int fd_move_model(fd_ctx* ctx, const float m[16], void* hip_stream);
+
+/* Words per receiver in the mask output for a res x res light grid. */
+int fd_mask_words(int light_res);
+
+/* Stage outputs of the last frame, for stage-by-stage verification:
+ * "rect" (8 floats per receiver), "key", "sorted_id", "bin_start", "bin_end",
+ * "bin_maxz", "bin_minrange", "tri_aabb", "tri_valid", "tri_data",
+ * "tri_count", "tri_range", "pair_tri", "bin_tri_start", "bin_tri_end",
+ * "light" (light block floats), "state" (raw state).  Synchronises the
+ * device.  *bytes receives the size; copies min(size, max_bytes). */
+int fd_debug_read(fd_ctx* ctx, const char* name, void* host_dst, size_t max_bytes, size_t* bytes);
+
+const char* fd_status_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FD4D_H */

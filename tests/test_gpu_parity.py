@@ -1,0 +1,137 @@
+"""HIP path vs the CPU oracle, stage by stage, bit-exact (MI355X only).
+
+Inputs are the deterministic workloads; every array the reference produces
+(light block, reductions, receiver rectangles, bin keys, the stable sort
+permutation, per-bin statistics, triangle records, pair counts, the sorted
+pair list, per-bin ranges, shadow values and light-sample masks) is compared
+by value / bit pattern.  At the full heightfield-1k and boxes-1080p sizes the
+occlusion is compared on a seeded receiver subset (the oracle is the cost);
+all other stages are compared in full.
+"""
+import numpy as np
+import pytest
+
+from parity import compare, failures
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(fd, oracle, frame, subset=None, expect_exit=None):
+    print(f"[parity] {frame.name} {frame.rx.w}x{frame.rx.h} light {frame.light.res_w} "
+          f"tris {frame.scene.ntris} grid {1 << frame.grid_log2}", flush=True)
+    rep, fr, o = compare(fd, oracle, frame, subset=subset)
+    bad = failures(rep)
+    assert not bad, f"{frame.name}: mismatches {bad} (oracle exit {o.exit})"
+    if expect_exit is not None:
+        assert o.exit == expect_exit, f"expected exit {expect_exit}, oracle gave {o.exit}"
+    fr.ctx.close()
+    return o
+
+
+def test_plane256_full(fd, oracle, wl):
+    o = _check(fd, oracle, wl.plane_256())
+    assert o.exit == 0 and o.scalars["pairs"] > 0
+
+
+@pytest.mark.parametrize("grid_log2", [3, 5, 6])
+def test_plane256_bin_grids(fd, oracle, wl, grid_log2):
+    fr = wl.plane_256(ntri=3000)
+    fr.grid_log2 = grid_log2
+    _check(fd, oracle, fr)
+
+
+@pytest.mark.parametrize("res", [1, 2, 5, 8, 31, 32, 33, 48, 64])
+def test_light_grids(fd, oracle, wl, res):
+    fr = wl.plane_256(ntri=2000, res=96, light_res=res)
+    _check(fd, oracle, fr)
+
+
+def test_non_pow2_view_and_cull_off(fd, oracle, wl):
+    fr = wl.plane_256(ntri=3000)
+    fr.rx = wl.receivers_heightfield(200, 120, 4.0, 2.4)
+    fr.backface_cull = 0
+    _check(fd, oracle, fr)
+
+
+def test_heightfield_small(fd, oracle, wl):
+    fr = wl.heightfield_1k(res=192, nsph=60, light_res=32)
+    _check(fd, oracle, fr)
+
+
+def test_heightfield_1k_subset(fd, oracle, wl):
+    fr = wl.heightfield_1k()
+    rng = np.random.default_rng(7)
+    sub = rng.choice(fr.rx.w * fr.rx.h, 6000, replace=False)
+    _check(fd, oracle, fr, subset=sub)
+
+
+def test_boxes_small_64(fd, oracle, wl):
+    fr = wl.boxes_1080p(w=320, h=180, ntri=60_000)
+    _check(fd, oracle, fr)
+
+
+def test_boxes_1080p_subset(fd, oracle, wl):
+    fr = wl.boxes_1080p()
+    rng = np.random.default_rng(11)
+    sub = rng.choice(fr.rx.w * fr.rx.h, 3000, replace=False)
+    _check(fd, oracle, fr, subset=sub)
+
+
+# --------------------------------------------------------------------------
+# early exits of FDLaunch (FD/api.cpp:495-559) and degenerate inputs
+# --------------------------------------------------------------------------
+def test_exit_no_receivers(fd, oracle, wl):
+    fr = wl.plane_256(ntri=500, res=64)
+    fr.rx.pos[...] = np.nan
+    _check(fd, oracle, fr, expect_exit=1)
+
+
+def test_exit_tris_behind_light(fd, oracle, wl):
+    fr = wl.plane_256(ntri=500, res=64)
+    fr.scene.verts[:, 2] += 10.0
+    _check(fd, oracle, fr, expect_exit=2)
+
+
+def test_exit_all_backfacing(fd, oracle, wl):
+    fr = wl.plane_256(ntri=500, res=64)
+    fr.rx.nrm[..., 2] = -1.0
+    _check(fd, oracle, fr, expect_exit=3)
+
+
+def test_exit_light_plane_empty(fd, oracle, wl):
+    fr = wl.plane_256(ntri=500, res=64)
+    fr.scene.verts[:, 0] += 60.0   # occluders far to the side
+    _check(fd, oracle, fr, expect_exit=5)
+
+
+def test_exit_no_valid_triangles(fd, oracle, wl):
+    fr = wl.plane_256(ntri=500, res=64)
+    # flatten every triangle to a horizontal one facing down, away from the light
+    v = fr.scene.verts.reshape(-1, 3, 4).copy()
+    v[:, :, 2] = v[:, :, 2].mean(axis=1, keepdims=True)
+    n = np.cross(v[:, 1, :3] - v[:, 0, :3], v[:, 2, :3] - v[:, 1, :3])
+    flip = n[:, 2] > 0
+    v[flip, 1], v[flip, 2] = v[flip, 2].copy(), v[flip, 1].copy()
+    fr.scene.verts[:] = v.reshape(-1, 4)
+    _check(fd, oracle, fr, expect_exit=6)
+
+
+def test_mixed_nan_and_backfacing(fd, oracle, wl):
+    fr = wl.plane_256(ntri=1500, res=128)
+    rng = np.random.default_rng(3)
+    m = rng.random((128, 128)) < 0.2
+    fr.rx.pos[m, 0] = np.nan
+    b = rng.random((128, 128)) < 0.2
+    fr.rx.nrm[b, 2] = -1.0
+    _check(fd, oracle, fr)
+
+
+def test_repeat_launch_idempotent(fd, oracle, wl):
+    """The reference divides its rectangle constant by f in place on every
+    FDLaunch (Step2_3_defineFDbuffer.inl:24-25); here launches repeat."""
+    fr = wl.plane_256(ntri=2000, res=128)
+    rep, run, o = compare(fd, oracle, fr)
+    assert not failures(rep)
+    rep2, _, _ = compare(fd, oracle, fr, fr=run, o=o)
+    assert not failures(rep2), failures(rep2)
+    run.ctx.close()

@@ -853,6 +853,33 @@ long long or_array(void* h, const char* name, void* dst) {
     return -1;
 }
 
+// Occlusion of the given receivers against EVERY triangle that reached the
+// light plane, with no binning (a brute-force cross-check of the binning).
+int or_occlude_all(void* h, const or_input* ci, const int* ids, int n, uint32_t* out) {
+    auto* R = (orc::Result*)h;
+    if (R->exitCode != orc::EXIT_OK) return -1;
+    std::vector<int> all;
+    for (int t = 0; t < (int)R->triValid.size(); ++t)
+        if (R->triValid[t]) all.push_back(t);
+    const int LH = R->light.resH;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int k = 0; k < n; ++k) {
+        int id = ids[k];
+        const float* r = &R->rect[(size_t)id * 8];
+        const float* p = ci->pos4 + 4 * (size_t)id;
+        orc::OccStats st;
+        uint64_t mk[64];
+        orc::occludeOne(R->light, ci->eps, orc::f3{p[0], p[1], p[2]}, orc::f4{r[0], r[1], r[4], r[5]}, r[6],
+                        all.data(), (int)all.size(), R->triData, mk, st);
+        uint32_t* o = out + (size_t)k * LH * R->wpr;
+        for (int rr = 0; rr < LH; ++rr) {
+            if (R->wpr == 1) o[rr] = (uint32_t)mk[rr];
+            else { o[2 * rr] = (uint32_t)mk[rr]; o[2 * rr + 1] = (uint32_t)(mk[rr] >> 32); }
+        }
+    }
+    return 0;
+}
+
 // The light-row mask primitive on its own (for closed-form tests).
 unsigned long long or_rowmask(float D, float B, unsigned n, int lightW, int* clip) {
     uint64_t FULL = (lightW >= 64) ? ~0ull : ((1ull << lightW) - 1ull);

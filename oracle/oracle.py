@@ -75,6 +75,9 @@ def lib():
         L.or_rowmask.restype = ctypes.c_ulonglong
         L.or_make_rectangle.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.or_max_threads.restype = ctypes.c_int
+        L.or_occlude_all.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Input), ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p]
+        L.or_occlude_all.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -115,6 +118,40 @@ class OracleResult:
                 self._h = None
         except Exception:
             pass
+
+
+def _make_input(frame):
+    sc, rx, L = frame.scene, frame.rx, frame.light
+    keep = dict(verts=np.ascontiguousarray(sc.verts, np.float32), idx=np.ascontiguousarray(sc.idx, np.int32),
+                pos=np.ascontiguousarray(rx.pos, np.float32), nrm=np.ascontiguousarray(rx.nrm, np.float32))
+    inp = _Input()
+    inp.verts = keep["verts"].ctypes.data
+    inp.vstride_floats = keep["verts"].shape[1]
+    inp.nverts = keep["verts"].shape[0]
+    inp.idx = keep["idx"].ctypes.data
+    inp.ntris = keep["idx"].shape[0]
+    inp.pos4 = keep["pos"].ctypes.data
+    inp.nrm4 = keep["nrm"].ctypes.data
+    inp.w = rx.w
+    inp.h = rx.h
+    inp.light = _Light(L.right_radius, L.top_radius, (ctypes.c_float * 3)(*L.position),
+                       (ctypes.c_float * 3)(*L.view_dir), (ctypes.c_float * 3)(*L.up_dir), L.res_w, L.res_h)
+    inp.eps = frame.eps
+    inp.backface_cull = frame.backface_cull
+    inp.grid_log2 = frame.grid_log2
+    return inp, keep
+
+
+def occlude_all(result, frame, ids):
+    """Masks of receivers `ids` against every light-plane triangle (no bins)."""
+    inp, keep = _make_input(frame)
+    ids = np.ascontiguousarray(ids, np.int32)
+    R = frame.light.res_w
+    out = np.zeros((len(ids), R * (1 if R <= 32 else 2)), np.uint32)
+    rc = lib().or_occlude_all(result._h, ctypes.byref(inp), ids.ctypes.data, len(ids), out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("occlude_all needs a frame that reached the occlusion stage")
+    return out
 
 
 def run(frame, subset=None, nthreads=0):

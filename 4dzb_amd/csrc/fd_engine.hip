@@ -124,7 +124,11 @@ struct fd_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     FdArgs last{};
     bool haveLast = false;
-    bool timed = false;
+    // profiling: stage events and occlusion op counters
+    bool profile = false;
+    bool countOps = false;
+    hipEvent_t stageEv[FD_NSTAGES + 1] = {};
+    DevBuf ops;
 };
 
 namespace {
@@ -191,6 +195,8 @@ int prepare(fd_ctx* c, FdArgs& a) {
     a.chunkStart = tb + 4 * (nb + 2);
     const long np = std::max(fdk::scan_partials_needed(nR), fdk::scan_partials_needed(nT)) + 1;
     a.scanPart = (uint64_t*)need(c->scanPart, np * sizeof(uint64_t));
+    a.opCount = nullptr;
+    if (c->countOps) a.opCount = (unsigned long long*)need(c->ops, 4 * sizeof(unsigned long long));
     if (c->masks) a.masks = c->masks;
     else a.masks = (uint32_t*)need(c->scratchMasks, (size_t)WH * R * wpr * sizeof(uint32_t));
     return hip_status(e);
@@ -257,6 +263,9 @@ int fd_destroy(fd_ctx* c) {
                       &c->triAABB, &c->tri, &c->triCount, &c->triRange, &c->histT, &c->sortedTri,
                       &c->tables, &c->scanPart, &c->scratchMasks};
     for (DevBuf* b : bufs) b->release();
+    c->ops.release();
+    for (hipEvent_t& ev : c->stageEv)
+        if (ev) (void)hipEventDestroy(ev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     delete c;
@@ -355,7 +364,9 @@ int fd_launch_async(fd_ctx* c, void* stream) {
     int rc = prepare(c, a);
     if (rc != FD_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = fdk::enqueue_frame(a, st);
+    hipError_t e = hipSuccess;
+    if (a.opCount) e = hipMemsetAsync(a.opCount, 0, 4 * sizeof(unsigned long long), st);
+    if (e == hipSuccess) e = fdk::enqueue_frame(a, st, c->profile ? c->stageEv : nullptr);
     c->last = a;
     c->haveLast = true;
     return hip_status(e);
@@ -409,6 +420,32 @@ int fd_move_model(fd_ctx* c, const float m[16], void* stream) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     return hip_status(e);
+}
+
+int fd_set_profiling(fd_ctx* c, int stage_events, int count_ops) {
+    if (!c) return FD_ERR_ARG;
+    if (stage_events && !c->stageEv[0])
+        for (hipEvent_t& ev : c->stageEv)
+            if (hipEventCreate(&ev) != hipSuccess) return FD_ERR_HIP;
+    c->profile = stage_events != 0;
+    c->countOps = count_ops != 0;
+    return FD_OK;
+}
+
+int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* ops4) {
+    if (!c || !c->haveLast) return FD_ERR_STATE;
+    if (stage_ms && c->profile) {
+        hipError_t e = hipEventSynchronize(c->stageEv[FD_NSTAGES]);
+        if (e != hipSuccess) return hip_status(e);
+        for (int i = 0; i < nstages && i < FD_NSTAGES; ++i)
+            (void)hipEventElapsedTime(stage_ms + i, c->stageEv[i], c->stageEv[i + 1]);
+    }
+    if (ops4) {
+        if (!c->last.opCount) return FD_ERR_STATE;
+        hipError_t e = hipMemcpy(ops4, c->last.opCount, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_status(e);
+    }
+    return FD_OK;
 }
 
 int fd_mask_words(int res) {

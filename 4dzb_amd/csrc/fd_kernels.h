@@ -50,9 +50,13 @@ struct FdArgs {
     int* triEnd;          // nb
     int* chunkStart;      // nb + 2
     uint64_t* scanPart;   // scan partial sums
+    unsigned long long* opCount;  // optional occlusion counters (4), or null
 };
 
+// Stage boundaries recorded when profiling (events[0..FD_NSTAGES]).
+#define FD_NSTAGES 5   // light plane, receiver binning, triangle pairing, tables, occlusion
+
 namespace fdk {
-hipError_t enqueue_frame(const FdArgs& a, hipStream_t st);
+hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* stageEvents = nullptr);
 long scan_partials_needed(long n);
 }  // namespace fdk

@@ -756,7 +756,7 @@ __device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds
     }
 }
 
-template <typename T>
+template <typename T, bool COUNT>
 __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 ids
     const FdState* s = a.st;
@@ -838,6 +838,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     const float eps = a.eps;
     T full = 0;
     const int ts = a.triStart[bin], te = a.triEnd[bin];
+    unsigned long long nCand = 0, nPass = 0, nRows = 0;
     // read-only for this kernel: the constant address space lets the uniform
     // per-triangle loads go through the scalar unit (s_load)
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -851,7 +852,9 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     const ctri_p tris = (ctri_p)(a.tri);
     for (int ti = ts; ti < te; ++ti) {
         const bool live = act && full != FULL;
-        if (__ballot(live) == 0) break;
+        const uint64_t liveMask = __ballot(live);
+        if (liveMask == 0) break;
+        if (COUNT) nCand += __popcll(liveMask);
         const int tid = triList[ti];
         if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
@@ -859,6 +862,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
         // containment + depth cull (:120-127)
         if (!(t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx && q.rmaxy <= t.bmaxy && t.minz <= q.z))
             continue;
+        if (COUNT) ++nPass;
         // dir = dot(Q - v0, n) (:130-132)
         const float dx = q.q[0] - t.v0[0], dy = q.q[1] - t.v0[1], dz = q.q[2] - t.v0[2];
         const float dir = dot3(dx, dy, dz, t.n[0], t.n[1], t.n[2]);
@@ -892,6 +896,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
         T valid = (~full) & N;
         // row by row, lowest row first (:204-222)
         while (valid) {
+            if (COUNT) ++nRows;
             const int r = Bits<T>::ctz(valid);
             const float fr = (float)r;
             T M = FULL;
@@ -921,6 +926,20 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
         if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
     }
     if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
+    if (COUNT) {
+        // per-lane counts: nCand is wave-wide already
+        unsigned long long p = nPass, r = nRows;
+        for (int o = 32; o >= 1; o >>= 1) {
+            p += __shfl_xor(p, o, 64);
+            r += __shfl_xor(r, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(a.opCount + 0, nCand);
+            atomicAdd(a.opCount + 1, p);
+            atomicAdd(a.opCount + 2, r);
+            atomicAdd(a.opCount + 3, 1ull);
+        }
+    }
     __syncthreads();
     store_masks<T>(a, lds, ids, n, R, wpr);
 }
@@ -966,20 +985,24 @@ hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st) {
     return e;
 }
 
-hipError_t enqueue_frame(const FdArgs& a, hipStream_t st) {
+hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     const int nb = 1 << (2 * a.gridLog2);
     const int nbR = cdiv(a.WH, RED_BLOCK * RED_ITEMS);
     const int nbV = cdiv(a.nverts, RED_BLOCK * RED_ITEMS);
     FdState* s = a.st;
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], st); };
+    mark(0);
     k_init<<<1, 64, 0, st>>>(a);
     k_init_bins<<<cdiv(nb, 256), 256, 0, st>>>(a);
     k_reduce<<<nbR + nbV, RED_BLOCK, 0, st>>>(a, nbR);
     k_finalize<<<1, 64, 0, st>>>(a);
+    mark(1);
     // receivers: key + bin statistics, stable counting sort
     k_rx_key<<<a.nblkR, 256, (nb + 1) * sizeof(uint32_t), st>>>(a);
     hipError_t e = scan_u32(a.histR, (long)(nb + 1) * a.nblkR, a.scanPart, &s->rxTotal, &s->status, st);
     if (e != hipSuccess) return e;
     k_rx_scatter<<<a.nblkR, 64, (nb + 1) * sizeof(int), st>>>(a);
+    mark(2);
     // triangles: setup + pair count, stable pair scatter
     if (a.ntris > 0) {
         k_tri_count<<<a.nblkT, FD_TRB, nb * sizeof(uint32_t), st>>>(a);
@@ -990,13 +1013,21 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st) {
     } else {
         k_post_pairs<<<1, 64, 0, st>>>(a);
     }
+    mark(3);
     k_tables<<<1, 1024, 0, st>>>(a);
+    mark(4);
     // occlusion (or the early-exit fill)
     const int chunks = cdiv(a.WH, 64) + nb + 1;
     const int wpr = a.L.wpr;
     const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * sizeof(int);
-    if (wpr == 1) k_occlude<uint32_t><<<chunks, 64, shm, st>>>(a);
-    else k_occlude<uint64_t><<<chunks, 64, shm, st>>>(a);
+    if (a.opCount) {
+        if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64, shm, st>>>(a);
+        else k_occlude<uint64_t, true><<<chunks, 64, shm, st>>>(a);
+    } else {
+        if (wpr == 1) k_occlude<uint32_t, false><<<chunks, 64, shm, st>>>(a);
+        else k_occlude<uint64_t, false><<<chunks, 64, shm, st>>>(a);
+    }
+    mark(5);
     return hipGetLastError();
 }
 

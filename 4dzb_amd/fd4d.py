@@ -84,6 +84,8 @@ def lib():
             "fd_mask_words": ([ip], ip),
             "fd_debug_read": ([vp, ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)], ip),
             "fd_status_string": ([ip], ctypes.c_char_p),
+            "fd_set_profiling": ([vp, ip, ip], ip),
+            "fd_get_profile": ([vp, ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_ulonglong)], ip),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -182,6 +184,15 @@ class Context:
     def move_model(self, m16, stream=None):
         arr = (ctypes.c_float * 16)(*[float(x) for x in m16])
         _check(lib().fd_move_model(self.h, arr, ctypes.c_void_p(stream or 0)), "fd_move_model")
+
+    def set_profiling(self, stage_events=True, count_ops=False):
+        _check(lib().fd_set_profiling(self.h, int(stage_events), int(count_ops)), "fd_set_profiling")
+
+    def get_profile(self, ops=False):
+        ms = (ctypes.c_float * 5)()
+        cnt = (ctypes.c_ulonglong * 4)()
+        _check(lib().fd_get_profile(self.h, ms, 5, cnt if ops else None), "fd_get_profile")
+        return list(ms), (list(cnt) if ops else None)
 
     def debug_read(self, name, dtype):
         n = ctypes.c_size_t()

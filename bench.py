@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Soft-shadow visibility benchmark on one MI355X.
+
+A "step" is one frame of the plain per-frame path (FD::FDLaunch's
+replacement) over the heightfield-1k workload: 1024x1024 receivers on
+z = 0.2 sin x cos y, a 2x2 area light at z = 6 sampled 32x32, and 200k
+occluder triangles (500 UV spheres).  Inputs are resident in HBM before the
+timed region; each frame writes the per-receiver shadow value and the
+1024-bit light-sample mask of every receiver.
+
+value = visibility Gbit/s = frames * W*H * Lw*Lh / seconds / 1e9.
+
+The roofline object describes the dominant kernel (the occlusion kernel):
+its duration comes from HIP events recorded around it on the launch stream,
+its work from the kernel's own counters (candidate tests, cull passes,
+visited light rows) priced with per-event operation counts of the
+reference's arithmetic.  cpu_baseline times the CPU restatement of the
+reference algorithm on this host (all cores) over a seeded receiver sample.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "visibility Gbit/s + p50/p99 frame ms, batched-served vs plain, bit-exact masks"
+# operations per counted event of the occlusion arithmetic
+# (Step5_2_shadowCalPerSample.inl:110-223): containment/depth cull, the
+# per-triangle planes + row-range masks, and one light row of three edge masks
+OPS_PER_CAND, OPS_PER_PASS, OPS_PER_ROW = 5, 160, 45
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pctl(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q))
+
+
+def gpu_plain(pkg, frame, steps, warmup):
+    import torch
+    fd = pkg.fd4d
+    run = fd.FrameRunner(frame)
+    ctx = run.ctx
+    stream = torch.cuda.current_stream().cuda_stream
+    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+    bits = W * H * R * R
+    rc, st = run.run()
+    if rc != 0:
+        raise RuntimeError(f"frame failed: rc {rc}")
+    for _ in range(warmup):
+        ctx.launch_async(stream)
+    torch.cuda.synchronize()
+    # throughput: frames enqueued back to back
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.launch_async(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    rc, st = ctx.get_stats(stream)
+    # latency: one frame at a time, host wall clock around fd_launch
+    lat = []
+    for _ in range(max(steps, 10)):
+        a = time.perf_counter()
+        rc, st = ctx.launch(stream)
+        lat.append((time.perf_counter() - a) * 1e3)
+    # stage times (HIP events around each stage on the launch stream)
+    ctx.set_profiling(True, False)
+    stage = []
+    for _ in range(5):
+        ctx.launch_async(stream)
+        stage.append(ctx.get_profile()[0])
+    stage = np.median(np.asarray(stage), axis=0)
+    # occlusion work counters (a separate frame: counting adds instructions)
+    ctx.set_profiling(False, True)
+    ctx.launch_async(stream)
+    _, ops = ctx.get_profile(ops=True)
+    ctx.set_profiling(False, False)
+    torch.cuda.synchronize()
+    shadow, masks = run.outputs()
+    return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=lat, stage=stage.tolist(), ops=ops,
+                stats=st.as_dict(), shadow=shadow, masks=masks)
+
+
+def cpu_baseline(frame, gpu, seconds):
+    """The oracle (CPU restatement of the reference) on a receiver sample,
+    all host cores; full-frame time = fixed stages + occlusion scaled."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+    nthreads = oracle.max_threads()
+    rng = np.random.default_rng(123)
+    n = 4096
+    o = oracle.run(frame, subset=rng.choice(W * H, n, replace=False), nthreads=nthreads)
+    per_rx = o.scalars["ms_occ"] / max(1, int(o.computed.sum()))
+    # grow the sample to fill about `seconds` of occlusion work
+    n2 = int(min(W * H, max(n, seconds * 1e3 / max(per_rx, 1e-6))))
+    sub = rng.choice(W * H, n2, replace=False)
+    t0 = time.perf_counter()
+    o = oracle.run(frame, subset=sub, nthreads=nthreads)
+    wall = time.perf_counter() - t0
+    s = o.scalars
+    computed = int(o.computed.sum())
+    fixed_ms = s["ms_lp"] + s["ms_setup"] + s["ms_bin"] + s["ms_pair"]
+    binned = int((o.key < int(s["num_bins"])).sum())
+    occ_full_ms = s["ms_occ"] * binned / max(1, computed)
+    frame_ms = fixed_ms + occ_full_ms
+    value = W * H * R * R / (frame_ms * 1e-3) / 1e9
+    ids = sub[o.computed[sub] > 0]
+    wpr = 1 if R <= 32 else 2
+    match = bool(np.array_equal(gpu["masks"].reshape(W * H, R * wpr)[ids], o.masks.reshape(W * H, R * wpr)[ids])
+                 and np.array_equal(gpu["shadow"][ids].view(np.uint32), o.shadow[ids].view(np.uint32)))
+    return dict(value=round(value, 6), unit="Gbit/s", cores=nthreads, kind="port",
+                sample=(f"oracle on {W}x{H} receivers / {frame.scene.ntris} triangles: all stages in full, "
+                        f"occlusion of {computed} sampled receivers ({wall:.1f} s wall), frame time = fixed "
+                        f"stages {fixed_ms:.0f} ms + occlusion scaled to {binned} binned receivers "
+                        f"({occ_full_ms:.0f} ms)"),
+                frame_ms=round(frame_ms, 2), masks_match_gpu_on_sample=match, sample_receivers=computed)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="heightfield-1k", choices=["heightfield-1k", "plane-256", "boxes-1080p"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
+              "without exchange; see the sharding design).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    pkg = importlib.import_module("4dzb_amd")
+    wl = pkg.workloads
+    log(f"[bench] generating {args.workload}")
+    frame = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
+        args.workload]()
+    log("[bench] GPU plain path")
+    g = gpu_plain(pkg, frame, args.steps, args.warmup)
+    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+    value = g["bits"] / (g["ms_per_step"] * 1e-3) / 1e9
+    occ_ms = g["stage"][4]
+    cand, npass, rows, items = g["ops"]
+    ops = OPS_PER_CAND * cand + OPS_PER_PASS * npass + OPS_PER_ROW * rows
+    achieved = ops / (occ_ms * 1e-3) / 1e12
+    roofline = dict(bound="valu", achieved=round(achieved, 3), peak=FP32_VECTOR_PEAK_TFLOPS, unit="TOP/s",
+                    frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), traffic=None,
+                    kernel="k_occlude", kernel_ms=round(occ_ms, 4),
+                    work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
+                              work_items=int(items), ops_per_event=[OPS_PER_CAND, OPS_PER_PASS, OPS_PER_ROW]))
+    out = dict(metric=METRIC, value=round(value, 4), unit="Gbit/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=round(g["ms_per_step"], 4), higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f32", data="synthetic (seeded generators, 4dzb_amd/csrc/scenes.cpp)",
+               config=dict(workload=args.workload, receivers=f"{W}x{H}", light_samples=f"{R}x{R}",
+                           triangles=frame.scene.ntris, bins=f"{1 << frame.grid_log2}x{1 << frame.grid_log2}",
+                           path="plain per-frame"),
+               p50_ms=round(pctl(g["lat"], 50), 3), p99_ms=round(pctl(g["lat"], 99), 3),
+               stage_ms=dict(zip(["light_plane", "receiver_binning", "triangle_pairing", "tables", "occlusion"],
+                                 [round(x, 4) for x in g["stage"]])),
+               frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
+                                valid_triangles=g["stats"]["valid_triangles"]),
+               roofline=roofline)
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

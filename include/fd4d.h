@@ -149,6 +149,14 @@ int fd_set_pair_capacity(fd_ctx* ctx, size_t max_pairs);
  * in-place 4x4 (column-major) transform of model 0's vertex positions. */
 int fd_move_model(fd_ctx* ctx, const float m[16], void* hip_stream);
 
+/* No counterpart (the reference times the whole FDLaunch with cudaEvents,
+ * FD/api.cpp:405-409,574-581): per-stage device times of the next frames
+ * (light plane, receiver binning, triangle pairing, tables, occlusion) and
+ * occlusion work counters (candidate tests, cull passes, visited rows,
+ * work items). */
+int fd_set_profiling(fd_ctx* ctx, int stage_events, int count_ops);
+int fd_get_profile(fd_ctx* ctx, float* stage_ms, int nstages, unsigned long long* ops4);
+
 /* Words per receiver in the mask output for a res x res light grid. */
 int fd_mask_words(int light_res);
 

@@ -716,17 +716,86 @@ struct Bits<uint64_t> {
 };
 
 // getRowMaskForPlaneDiscriminant (:14-33): the mask of light samples along
-// one row (or of rows) whose plane value is non-negative.  Bn = B * n is
-// passed precomputed (the reference evaluates B * n + D).  Shifts outside
-// [0, width) give 0, conversions saturate; the caller clips to the grid.
+// one row (or of rows) whose plane value D + B*c is non-negative, as
+//   f1 = D >= 0, f2 = B*n + D >= 0;  f1 == f2 -> all / none,
+//   else k = (int)ceil(-1*D/B), mask = (1 << k) - 1, complemented if !f1.
+// Only min(k, R) is observable once the mask is clipped to the R-sample
+// grid, and k outside [0, width) (shift to zero, then -1) is all ones, so k
+// is reduced to kk in [0, R] with every out-of-grid value mapped to R.
+//
+// The division is the hot operation.  q' = -D * rcp(B) (v_rcp_f32, 1 ulp)
+// is within |q| * 2^-22 of the correctly rounded quotient q, i.e. within
+// 2e-5 for q <= 80.  Whenever q' is at least 2^-12 away from every integer,
+// ceil(q') == ceil(q); q' >= R + 1 implies q > R.  Only the remaining cases
+// (near an integer, near zero, negative, or |B| outside [2^-100, 2^100])
+// take the IEEE division.
+__device__ __forceinline__ int k_exact(float D, float B, int R) {
+    const float c = ceilf(-1.0f * D / B);
+    if (c != c) return 0;                       // cvt.rzi of NaN
+    if (c < 0.0f || c > (float)R) return R;     // shift out of range / beyond the grid
+    return (int)c;
+}
+
+struct Edge {
+    float A, B, C;    // plane value A*row + B*col + C
+    float rowN;       // A * (R-1)   (f2 of the row-range mask)
+    float colN;       // B * (R-1)   (f2 of a row mask)
+    float rA, rB;     // reciprocals for the fast quotient
+    bool okA, okB;    // |A| / |B| inside [2^-100, 2^100]
+};
+
+__device__ __forceinline__ bool rcp_ok(float x) {
+    const float ax = fabsf(x);
+    return ax >= 7.8886091e-31f && ax <= 1.2676506e+30f;
+}
+
+// kk for one (D, divisor) in the division branch; sets slow when the exact
+// quotient is needed
+__device__ __forceinline__ int k_fast(float D, float r, bool ok, float Rf, int R, bool& slow) {
+    const float q = (-D) * r;
+    const float dist = fabsf(q - rintf(q));
+    const bool big = ok && q >= Rf + 1.0f;
+    const bool good = ok && q >= 0.0f && dist > 2.44140625e-4f;
+    slow = !(big || good);
+    const float c = fmaxf(fminf(ceilf(q), Rf), 0.0f);   // finite, in [0, R]
+    return big ? R : (int)c;
+}
+
 template <typename T>
-__device__ __forceinline__ T row_mask(float D, float B, float Bn) {
-    const bool f1 = (D >= 0.0f);
-    const bool f2 = (Bn + D >= 0.0f);
-    if (f1 == f2) return f1 ? ~(T)0 : (T)0;
-    const int k = cvt_sat(ceilf(-1.0f * D / B));
-    T m = (k >= 0 && k < Bits<T>::width) ? (((T)1 << k) - (T)1) : ~(T)0;
-    return f1 ? m : ~m;
+__device__ __forceinline__ T low_mask(int kk) {
+    return (kk >= Bits<T>::width) ? ~(T)0 : (((T)1 << kk) - (T)1);
+}
+
+// mask of one edge: f1/f2 as in the reference, kk the reduced shift
+template <typename T>
+__device__ __forceinline__ T edge_mask(bool f1, bool f2, int kk) {
+    const T lo = low_mask<T>(kk);
+    const T part = f1 ? lo : ~lo;
+    const T whole = f1 ? ~(T)0 : (T)0;
+    return (f1 == f2) ? whole : part;
+}
+
+// AND of the three edge masks at plane offsets D[k] (row masks: divisor B,
+// row-range masks: divisor A)
+template <typename T, bool ROWS>
+__device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3], float Rf, int R) {
+    bool f1[3], f2[3], slow[3];
+    int kk[3];
+    bool anySlow = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f1[k] = D[k] >= 0.0f;
+        f2[k] = ((ROWS ? e[k].rowN : e[k].colN) + D[k]) >= 0.0f;
+        kk[k] = k_fast(D[k], ROWS ? e[k].rA : e[k].rB, ROWS ? e[k].okA : e[k].okB, Rf, R, slow[k]);
+        slow[k] = slow[k] && (f1[k] != f2[k]);
+        anySlow |= slow[k];
+    }
+    if (anySlow) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (slow[k]) kk[k] = k_exact(D[k], ROWS ? e[k].A : e[k].B, R);
+    }
+    return edge_mask<T>(f1[0], f2[0], kk[0]) & edge_mask<T>(f1[1], f2[1], kk[1]) & edge_mask<T>(f1[2], f2[2], kk[2]);
 }
 
 // test3DPointInTriangle, Step5_2_shadowCal.inl:213-222
@@ -835,6 +904,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     const float xd0 = a.L.xd[0], xd1 = a.L.xd[1], xd2 = a.L.xd[2];
     const float yd0 = a.L.yd[0], yd1 = a.L.yd[1], yd2 = a.L.yd[2];
     const float nsub = (float)(unsigned)(R - 1);
+    const float Rf = (float)R;
     const float eps = a.eps;
     T full = 0;
     const int ts = a.triStart[bin], te = a.triEnd[bin];
@@ -871,7 +941,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
         // negates t0, t1, L0_Q when dir < 0; negation commutes with every
         // rounding here, so A, B, C are negated instead (identical values up
         // to the sign of zero results, which no later step can observe).
-        float A[3], Bv[3], C[3];
+        Edge e[3];
         const float* vb[3] = {t.v1, t.v2, t.v0};
         const float* ev[3] = {t.e01, t.e12, t.e20};
 #pragma unroll
@@ -879,29 +949,32 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
             const float ax = vb[k][0] - q.q[0], ay = vb[k][1] - q.q[1], az = vb[k][2] - q.q[2];
             const float bx = ev[k][0], by = ev[k][1], bz = ev[k][2];
             const float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-            A[k] = dot3(nx, ny, nz, xd0, xd1, xd2);
-            Bv[k] = dot3(nx, ny, nz, yd0, yd1, yd2);
-            C[k] = dot3(nx, ny, nz, Lx, Ly, Lz);
-            if (dir < 0) { A[k] = -A[k]; Bv[k] = -Bv[k]; C[k] = -C[k]; }
+            float A = dot3(nx, ny, nz, xd0, xd1, xd2);
+            float B = dot3(nx, ny, nz, yd0, yd1, yd2);
+            float C = dot3(nx, ny, nz, Lx, Ly, Lz);
+            if (dir < 0) { A = -A; B = -B; C = -C; }
+            e[k].A = A; e[k].B = B; e[k].C = C;
+            e[k].rowN = A * nsub;
+            e[k].colN = B * nsub;
+            e[k].rA = __builtin_amdgcn_rcpf(A);
+            e[k].rB = __builtin_amdgcn_rcpf(B);
+            e[k].okA = rcp_ok(A);
+            e[k].okB = rcp_ok(B);
         }
         // rows that may hold occluded samples (:176-191)
-        float Bn[3];
-        T N = FULL;
+        float DN[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            Bn[k] = Bv[k] * nsub;
-            const float D = (Bv[k] > 0 ? Bn[k] : 0.0f) + C[k];
-            N &= row_mask<T>(D, A[k], A[k] * nsub);
-        }
-        T valid = (~full) & N;
+        for (int k = 0; k < 3; ++k) DN[k] = (e[k].B > 0 ? e[k].colN : 0.0f) + e[k].C;
+        T valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
         // row by row, lowest row first (:204-222)
         while (valid) {
             if (COUNT) ++nRows;
             const int r = Bits<T>::ctz(valid);
             const float fr = (float)r;
-            T M = FULL;
+            float D[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) M &= row_mask<T>(A[k] * fr + C[k], Bv[k], Bn[k]);
+            for (int k = 0; k < 3; ++k) D[k] = e[k].A * fr + e[k].C;
+            const T M = FULL & three_edges<T, false>(e, D, Rf, R);
             uint32_t* w0 = lds + r * 64 + lane;
             T cur;
             if (Bits<T>::width == 32) {
@@ -944,6 +1017,27 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     store_masks<T>(a, lds, ids, n, R, wpr);
 }
 
+// Verification of the row-mask primitive on given (D, B) pairs (n = R - 1),
+// through the same fast / exact paths as the occlusion kernel.
+__global__ void k_selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Edge e[3];
+    float DD[3];
+    for (int k = 0; k < 3; ++k) {
+        e[k].A = 0.0f; e[k].B = B[i]; e[k].C = 0.0f;
+        e[k].rowN = 0.0f;
+        e[k].colN = B[i] * (float)(unsigned)(R - 1);
+        e[k].rA = 0.0f; e[k].okA = false;
+        e[k].rB = __builtin_amdgcn_rcpf(B[i]);
+        e[k].okB = rcp_ok(B[i]);
+        DD[k] = D[i];
+    }
+    const uint64_t FULL = (R >= 64) ? ~0ull : ((1ull << R) - 1ull);
+    if (R <= 32) out[i] = (uint64_t)(three_edges<uint32_t, false>(e, DD, (float)R, R)) & FULL;
+    else out[i] = three_edges<uint64_t, false>(e, DD, (float)R, R) & FULL;
+}
+
 // Stage read-back of the receiver rectangles in the reference's 2-layer
 // layout (Step2_1_samplerRectangleAABB.inl:122-123); verification only.
 __global__ void k_debug_rect(FdArgs a, float* out) {
@@ -977,6 +1071,13 @@ static hipError_t scan_u32(uint32_t* d, long n, uint64_t* part, unsigned long lo
 }
 
 long scan_partials_needed(long n) { return cdiv(n, SCAN_CHUNK); }
+
+hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out) {
+    k_selftest_rowmask<<<cdiv(n, 256), 256>>>(D, B, n, R, out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+}
 
 hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st) {
     k_debug_rect<<<cdiv(a.WH, 256), 256, 0, st>>>(a, out8);

@@ -85,6 +85,7 @@ def lib():
             "fd_debug_read": ([vp, ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)], ip),
             "fd_status_string": ([ip], ctypes.c_char_p),
             "fd_set_profiling": ([vp, ip, ip], ip),
+            "fd_selftest_rowmask": ([vp, vp, ip, ip, vp], ip),
             "fd_get_profile": ([vp, ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_ulonglong)], ip),
         }
         for name, (args, res) in sig.items():
@@ -97,6 +98,16 @@ def lib():
 
 def mask_words(res):
     return lib().fd_mask_words(res)
+
+
+def selftest_rowmask(D, B, res):
+    """Device row-mask primitive on host arrays (verification only)."""
+    D = np.ascontiguousarray(D, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    out = np.empty(len(D), np.uint64)
+    _check(lib().fd_selftest_rowmask(D.ctypes.data, B.ctypes.data, len(D), res, out.ctypes.data),
+           "fd_selftest_rowmask")
+    return out
 
 
 class FdError(RuntimeError):

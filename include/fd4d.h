@@ -168,6 +168,11 @@ int fd_mask_words(int light_res);
  * device.  *bytes receives the size; copies min(size, max_bytes). */
 int fd_debug_read(fd_ctx* ctx, const char* name, void* host_dst, size_t max_bytes, size_t* bytes);
 
+/* Verification only: evaluates the light-row mask primitive of the
+ * occlusion kernel (Step5_2_shadowCalPerSample.inl:14-33, clipped to a
+ * res-sample row) on n host-given (D, B) pairs, one 64-bit mask each. */
+int fd_selftest_rowmask(const float* D, const float* B, int n, int res, unsigned long long* out);
+
 const char* fd_status_string(int code);
 
 #ifdef __cplusplus

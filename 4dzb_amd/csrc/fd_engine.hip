@@ -448,6 +448,39 @@ int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* 
     return FD_OK;
 }
 
+// ---- device buffers and streams for callers without their own HIP plumbing
+int fd_device_count(int* n) {
+    if (!n) return FD_ERR_ARG;
+    *n = 0;
+    if (hipGetDeviceCount(n) != hipSuccess) { *n = 0; return FD_ERR_NO_DEVICE; }
+    return *n > 0 ? FD_OK : FD_ERR_NO_DEVICE;
+}
+int fd_dev_alloc(size_t bytes, void** out) {
+    if (!out) return FD_ERR_ARG;
+    *out = nullptr;
+    return hip_status(hipMalloc(out, bytes ? bytes : 16));
+}
+int fd_dev_free(void* p) { return p ? hip_status(hipFree(p)) : FD_OK; }
+int fd_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    if (!dst || (!src && bytes)) return FD_ERR_ARG;
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+int fd_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    if (!dst || (!src && bytes)) return FD_ERR_ARG;
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+int fd_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || (!src && bytes)) return FD_ERR_ARG;
+    return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+}
+int fd_stream_create(void** out) {
+    if (!out) return FD_ERR_ARG;
+    return hip_status(hipStreamCreateWithFlags((hipStream_t*)out, hipStreamNonBlocking));
+}
+int fd_stream_destroy(void* s) { return s ? hip_status(hipStreamDestroy((hipStream_t)s)) : FD_OK; }
+int fd_stream_sync(void* s) { return hip_status(hipStreamSynchronize((hipStream_t)s)); }
+int fd_device_sync(void) { return hip_status(hipDeviceSynchronize()); }
+
 int fd_mask_words(int res) {
     if (res < 1 || res > 64) return FD_ERR_ARG;
     return res * (res <= 32 ? 1 : 2);

@@ -258,7 +258,8 @@ __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
         const int i = base + it * 256 + threadIdx.x;
         int key = nb;
         RxRect q{};
-        if (i < a.WH) {
+        const bool in = i < a.WH;
+        if (in) {
             const float4 r = a.rec[i];
             if (__float_as_uint(r.w) == FD_TAG_VALID) {
                 q = rx_rect(r, s, a.L);
@@ -268,36 +269,36 @@ __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
                 if (oor) ++noor;
             }
             a.rxKey[i] = (uint16_t)key;
-            atomicAdd(&hist[key], 1u);
         }
-        // per-bin statistics: wave-aggregate when the wave shares one key
-        const bool has = (i < a.WH) && key < nb;
-        const uint64_t hasm = __ballot(has);
-        const int k0 = hasm ? uniform(__shfl(key, __ffsll((unsigned long long)hasm) - 1, 64)) : -1;
-        const uint64_t same = __ballot(has && key == k0);
-        uint32_t v0 = has ? fenc(q.z) : 0u, v1 = has ? fenc(q.bx) : 0u, v2 = has ? fenc(q.by) : 0u;
-        uint32_t v3 = has ? fenc(q.ex) : 0xffffffffu, v4 = has ? fenc(q.ey) : 0xffffffffu;
-        // NaN never reaches a bin: a NaN rectangle fails no comparison, but its
-        // centre floors to bin 0; keep NaN out of the ordered encodings.
-        if (has) {
-            if (q.z != q.z) v0 = 0u;
-            if (q.bx != q.bx) v1 = 0u;
-            if (q.by != q.by) v2 = 0u;
-            if (q.ex != q.ex) v3 = 0xffffffffu;
-            if (q.ey != q.ey) v4 = 0xffffffffu;
-        }
-        if (hasm != 0 && same == hasm) {
-            v0 = wave_max_u32(v0); v1 = wave_max_u32(v1); v2 = wave_max_u32(v2);
-            v3 = wave_min_u32(v3); v4 = wave_min_u32(v4);
-            if (lane_id() == 0) {
-                uint32_t* bs = a.binStat + 5 * k0;
-                atomicMax(bs + 0, v0); atomicMax(bs + 1, v1); atomicMax(bs + 2, v2);
-                atomicMin(bs + 3, v3); atomicMin(bs + 4, v4);
+        // Segmented wave reduction: one pass per distinct key among the
+        // wave's lanes (receivers of a wave are neighbours, so usually 1-3),
+        // giving the key histogram and the per-bin receiver statistics with
+        // one LDS add and five global atomics per (wave, key).
+        const uint32_t v0 = (in && q.z == q.z) ? fenc(q.z) : 0u;
+        const uint32_t v1 = (in && q.bx == q.bx) ? fenc(q.bx) : 0u;
+        const uint32_t v2 = (in && q.by == q.by) ? fenc(q.by) : 0u;
+        const uint32_t v3 = (in && q.ex == q.ex) ? fenc(q.ex) : 0xffffffffu;
+        const uint32_t v4 = (in && q.ey == q.ey) ? fenc(q.ey) : 0xffffffffu;
+        uint64_t pending = __ballot(in);
+        while (pending) {
+            const int lead = __ffsll((unsigned long long)pending) - 1;
+            const int k0 = uniform(__shfl(key, lead, 64));
+            const bool mine = in && key == k0;
+            const uint64_t m = __ballot(mine);
+            pending &= ~m;
+            if (lane_id() == 0) atomicAdd(&hist[k0], (uint32_t)__popcll(m));
+            if (k0 < nb) {
+                const uint32_t r0 = wave_max_u32(mine ? v0 : 0u);
+                const uint32_t r1 = wave_max_u32(mine ? v1 : 0u);
+                const uint32_t r2 = wave_max_u32(mine ? v2 : 0u);
+                const uint32_t r3 = wave_min_u32(mine ? v3 : 0xffffffffu);
+                const uint32_t r4 = wave_min_u32(mine ? v4 : 0xffffffffu);
+                if (lane_id() == 0) {
+                    uint32_t* bs = a.binStat + 5 * k0;
+                    atomicMax(bs + 0, r0); atomicMax(bs + 1, r1); atomicMax(bs + 2, r2);
+                    atomicMin(bs + 3, r3); atomicMin(bs + 4, r4);
+                }
             }
-        } else if (has) {
-            uint32_t* bs = a.binStat + 5 * key;
-            atomicMax(bs + 0, v0); atomicMax(bs + 1, v1); atomicMax(bs + 2, v2);
-            atomicMin(bs + 3, v3); atomicMin(bs + 4, v4);
         }
     }
     nvalid = wave_sum_i32(nvalid);

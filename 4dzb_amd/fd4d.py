@@ -1,8 +1,9 @@
 """ctypes binding of libfd4d.so (include/fd4d.h).
 
-Device buffers are passed as raw device pointers; the helpers below take
-torch CUDA tensors for convenience (torch is only used as a device-memory
-and stream provider).  The library must be present: there is no fallback.
+Device buffers are raw device pointers.  `DeviceBuffer` and `Stream` wrap
+the library's own allocation / stream helpers, so a process that drives the
+library needs no other GPU runtime.  The library must be present: there is
+no fallback.
 """
 from __future__ import annotations
 
@@ -86,6 +87,16 @@ def lib():
             "fd_status_string": ([ip], ctypes.c_char_p),
             "fd_set_profiling": ([vp, ip, ip], ip),
             "fd_selftest_rowmask": ([vp, vp, ip, ip, vp], ip),
+            "fd_device_count": ([ctypes.POINTER(ip)], ip),
+            "fd_dev_alloc": ([sz, ctypes.POINTER(vp)], ip),
+            "fd_dev_free": ([vp], ip),
+            "fd_memcpy_htod": ([vp, vp, sz], ip),
+            "fd_memcpy_dtoh": ([vp, vp, sz], ip),
+            "fd_memcpy_dtod": ([vp, vp, sz, vp], ip),
+            "fd_stream_create": ([ctypes.POINTER(vp)], ip),
+            "fd_stream_destroy": ([vp], ip),
+            "fd_stream_sync": ([vp], ip),
+            "fd_device_sync": ([], ip),
             "fd_get_profile": ([vp, ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_ulonglong)], ip),
         }
         for name, (args, res) in sig.items():
@@ -120,7 +131,76 @@ def _check(rc, what):
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+    return ctypes.c_void_p(t.ptr if t is not None else 0)
+
+
+def device_count():
+    n = ctypes.c_int()
+    lib().fd_device_count(ctypes.byref(n))
+    return n.value
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (freed on close / garbage collection)."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().fd_dev_alloc(int(nbytes), ctypes.byref(p)), "fd_dev_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().fd_memcpy_htod(ctypes.c_void_p(self.ptr), ctypes.c_void_p(a.ctypes.data), a.nbytes),
+               "fd_memcpy_htod")
+
+    def download(self, dtype, count=None):
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dt)
+        _check(lib().fd_memcpy_dtoh(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.ptr), out.nbytes),
+               "fd_memcpy_dtoh")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().fd_dev_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().fd_stream_create(ctypes.byref(h)), "fd_stream_create")
+        self.handle = h.value
+
+    def sync(self):
+        _check(lib().fd_stream_sync(ctypes.c_void_p(self.handle)), "fd_stream_sync")
+
+    def close(self):
+        if self.handle:
+            lib().fd_stream_destroy(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -157,16 +237,14 @@ class Context:
     def set_pair_capacity(self, n):
         _check(lib().fd_set_pair_capacity(self.h, n), "fd_set_pair_capacity")
 
-    def add_model(self, verts_dev, idx_dev):
-        """verts_dev: (V, k>=3) float32 CUDA tensor; idx_dev: (T, 3) int32."""
+    def add_model(self, verts_dev, nverts, stride_bytes, idx_dev, ntris):
+        """verts_dev / idx_dev: DeviceBuffer of float vertices / int3 indices."""
         lib().fd_clear_models(self.h)
-        stride = verts_dev.shape[1] * 4
-        _check(lib().fd_add_model(self.h, _ptr(verts_dev), stride, verts_dev.shape[0], _ptr(idx_dev),
-                                  idx_dev.shape[0]), "fd_add_model")
+        _check(lib().fd_add_model(self.h, _ptr(verts_dev), stride_bytes, nverts, _ptr(idx_dev), ntris),
+               "fd_add_model")
         self._keep += [verts_dev, idx_dev]
 
-    def set_receivers(self, pos_dev, nrm_dev):
-        h, w = pos_dev.shape[0], pos_dev.shape[1]
+    def set_receivers(self, pos_dev, nrm_dev, w, h):
         _check(lib().fd_set_view(self.h, w, h), "fd_set_view")
         _check(lib().fd_set_sample_buffers(self.h, _ptr(pos_dev), _ptr(nrm_dev)), "fd_set_sample_buffers")
         self._keep += [pos_dev, nrm_dev]
@@ -218,20 +296,19 @@ class Context:
 class FrameRunner:
     """Uploads a workloads.Frame to the GPU and runs it through one Context."""
 
-    def __init__(self, frame, with_masks=True, ctx=None, pair_capacity=None):
-        import torch
-        self.torch = torch
+    def __init__(self, frame, with_masks=True, ctx=None, pair_capacity=None, stream=None):
         self.frame = frame
-        dev = torch.device("cuda", 0)
-        self.verts = torch.from_numpy(frame.scene.verts).to(dev)
-        self.idx = torch.from_numpy(frame.scene.idx).to(dev)
-        self.pos = torch.from_numpy(frame.rx.pos).to(dev)
-        self.nrm = torch.from_numpy(frame.rx.nrm).to(dev)
-        W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
-        self.shadow = torch.empty(H * W, dtype=torch.float32, device=dev)
+        sc, rx = frame.scene, frame.rx
+        self.verts = DeviceBuffer.from_array(np.ascontiguousarray(sc.verts, np.float32))
+        self.idx = DeviceBuffer.from_array(np.ascontiguousarray(sc.idx, np.int32))
+        self.pos = DeviceBuffer.from_array(np.ascontiguousarray(rx.pos, np.float32))
+        self.nrm = DeviceBuffer.from_array(np.ascontiguousarray(rx.nrm, np.float32))
+        W, H, R = rx.w, rx.h, frame.light.res_w
+        self.shadow = DeviceBuffer(H * W * 4)
         self.masks = None
-        if with_masks and 1 <= R <= 64:
-            self.masks = torch.empty(H * W * mask_words(R), dtype=torch.int32, device=dev)
+        if with_masks and 1 <= R <= 64 and frame.light.res_h == R:
+            self.masks = DeviceBuffer(H * W * mask_words(R) * 4)
+        self.stream = stream
         self.ctx = ctx or Context()
         c = self.ctx
         c.set_backface_cull(frame.backface_cull)
@@ -239,19 +316,28 @@ class FrameRunner:
         c.set_bin_num(1 << frame.grid_log2)
         if pair_capacity:
             c.set_pair_capacity(pair_capacity)
-        c.add_model(self.verts, self.idx)
-        c.set_receivers(self.pos, self.nrm)
+        c.add_model(self.verts, sc.nverts, sc.verts.shape[1] * 4, self.idx, sc.ntris)
+        c.set_receivers(self.pos, self.nrm, W, H)
         c.set_outputs(self.shadow, self.masks)
         self.light_rc = lib().fd_set_area_light(c.h, ctypes.byref(AreaLight.from_desc(frame.light)))
 
+    @property
+    def stream_handle(self):
+        return self.stream.handle if self.stream is not None else None
+
     def run(self):
-        rc, st = self.ctx.launch(self.torch.cuda.current_stream().cuda_stream)
-        return rc, st
+        return self.ctx.launch(self.stream_handle)
 
     def outputs(self):
-        sh = self.shadow.cpu().numpy()
-        mk = self.masks.cpu().numpy().view(np.uint32) if self.masks is not None else None
+        sh = self.shadow.download(np.float32)
+        mk = self.masks.download(np.uint32) if self.masks is not None else None
         return sh, mk
+
+    def close(self):
+        self.ctx.close()
+        for b in (self.verts, self.idx, self.pos, self.nrm, self.shadow, self.masks):
+            if b is not None:
+                b.close()
 
 
 # numpy mirrors of fd_types.h records (for stage read-back)

@@ -47,11 +47,11 @@ def pctl(xs, q):
 
 
 def gpu_plain(pkg, frame, steps, warmup):
-    import torch
     fd = pkg.fd4d
-    run = fd.FrameRunner(frame)
+    strm = fd.Stream()
+    run = fd.FrameRunner(frame, stream=strm)
     ctx = run.ctx
-    stream = torch.cuda.current_stream().cuda_stream
+    stream = strm.handle
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
     bits = W * H * R * R
     rc, st = run.run()
@@ -59,12 +59,12 @@ def gpu_plain(pkg, frame, steps, warmup):
         raise RuntimeError(f"frame failed: rc {rc}")
     for _ in range(warmup):
         ctx.launch_async(stream)
-    torch.cuda.synchronize()
+    strm.sync()
     # throughput: frames enqueued back to back
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.launch_async(stream)
-    torch.cuda.synchronize()
+    strm.sync()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / steps
     rc, st = ctx.get_stats(stream)
@@ -86,7 +86,7 @@ def gpu_plain(pkg, frame, steps, warmup):
     ctx.launch_async(stream)
     _, ops = ctx.get_profile(ops=True)
     ctx.set_profiling(False, False)
-    torch.cuda.synchronize()
+    strm.sync()
     shadow, masks = run.outputs()
     return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=lat, stage=stage.tolist(), ops=ops,
                 stats=st.as_dict(), shadow=shadow, masks=masks)
@@ -141,11 +141,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
               "without exchange; see the sharding design).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    import torch
-    if not torch.cuda.is_available():
+    pkg = importlib.import_module("4dzb_amd")
+    if pkg.fd4d.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
-    pkg = importlib.import_module("4dzb_amd")
     wl = pkg.workloads
     log(f"[bench] generating {args.workload}")
     frame = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[

@@ -157,6 +157,20 @@ int fd_move_model(fd_ctx* ctx, const float m[16], void* hip_stream);
 int fd_set_profiling(fd_ctx* ctx, int stage_events, int count_ops);
 int fd_get_profile(fd_ctx* ctx, float* stage_ms, int nstages, unsigned long long* ops4);
 
+/* No counterpart (the reference receives GL objects): device memory and
+ * streams for callers without their own HIP allocator.  Copies are
+ * synchronous except fd_memcpy_dtod. */
+int fd_device_count(int* n);
+int fd_dev_alloc(size_t bytes, void** out);
+int fd_dev_free(void* p);
+int fd_memcpy_htod(void* dst, const void* src, size_t bytes);
+int fd_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int fd_memcpy_dtod(void* dst, const void* src, size_t bytes, void* hip_stream);
+int fd_stream_create(void** out);
+int fd_stream_destroy(void* hip_stream);
+int fd_stream_sync(void* hip_stream);
+int fd_device_sync(void);
+
 /* Words per receiver in the mask output for a res x res light grid. */
 int fd_mask_words(int light_res);
 

@@ -24,7 +24,7 @@ def _check(fd, oracle, frame, subset=None, expect_exit=None):
     assert not bad, f"{frame.name}: mismatches {bad} (oracle exit {o.exit})"
     if expect_exit is not None:
         assert o.exit == expect_exit, f"expected exit {expect_exit}, oracle gave {o.exit}"
-    fr.ctx.close()
+    fr.close()
     return o
 
 
@@ -134,4 +134,4 @@ def test_repeat_launch_idempotent(fd, oracle, wl):
     assert not failures(rep)
     rep2, _, _ = compare(fd, oracle, fr, fr=run, o=o)
     assert not failures(rep2), failures(rep2)
-    run.ctx.close()
+    run.close()

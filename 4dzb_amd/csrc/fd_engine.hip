@@ -196,7 +196,7 @@ int prepare(fd_ctx* c, FdArgs& a) {
     const long np = std::max(fdk::scan_partials_needed(nR), fdk::scan_partials_needed(nT)) + 1;
     a.scanPart = (uint64_t*)need(c->scanPart, np * sizeof(uint64_t));
     a.opCount = nullptr;
-    if (c->countOps) a.opCount = (unsigned long long*)need(c->ops, 4 * sizeof(unsigned long long));
+    if (c->countOps) a.opCount = (unsigned long long*)need(c->ops, FD_NOPS * sizeof(unsigned long long));
     if (c->masks) a.masks = c->masks;
     else a.masks = (uint32_t*)need(c->scratchMasks, (size_t)WH * R * wpr * sizeof(uint32_t));
     return hip_status(e);
@@ -365,7 +365,7 @@ int fd_launch_async(fd_ctx* c, void* stream) {
     if (rc != FD_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSuccess;
-    if (a.opCount) e = hipMemsetAsync(a.opCount, 0, 4 * sizeof(unsigned long long), st);
+    if (a.opCount) e = hipMemsetAsync(a.opCount, 0, FD_NOPS * sizeof(unsigned long long), st);
     if (e == hipSuccess) e = fdk::enqueue_frame(a, st, c->profile ? c->stageEv : nullptr);
     c->last = a;
     c->haveLast = true;
@@ -433,6 +433,7 @@ int fd_set_profiling(fd_ctx* c, int stage_events, int count_ops) {
 }
 
 int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* ops4) {
+    // ops4 receives FD_NOPS counters
     if (!c || !c->haveLast) return FD_ERR_STATE;
     if (stage_ms && c->profile) {
         hipError_t e = hipEventSynchronize(c->stageEv[FD_NSTAGES]);
@@ -442,7 +443,9 @@ int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* 
     }
     if (ops4) {
         if (!c->last.opCount) return FD_ERR_STATE;
-        hipError_t e = hipMemcpy(ops4, c->last.opCount, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+        hipError_t e = hipDeviceSynchronize();   // the frame may run on a non-blocking stream
+        if (e == hipSuccess)
+            e = hipMemcpy(ops4, c->last.opCount, FD_NOPS * sizeof(unsigned long long), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_status(e);
     }
     return FD_OK;

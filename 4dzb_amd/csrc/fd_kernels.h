@@ -50,8 +50,12 @@ struct FdArgs {
     int* triEnd;          // nb
     int* chunkStart;      // nb + 2
     uint64_t* scanPart;   // scan partial sums
-    unsigned long long* opCount;  // optional occlusion counters (4), or null
+    unsigned long long* opCount;  // optional occlusion counters (FD_NOPS), or null
 };
+
+// Occlusion counters: lane candidate tests, lane cull passes, lane row
+// visits, work items, wave triangle iterations, wave row iterations.
+#define FD_NOPS 6
 
 // Stage boundaries recorded when profiling (events[0..FD_NSTAGES]).
 #define FD_NSTAGES 5   // light plane, receiver binning, triangle pairing, tables, occlusion

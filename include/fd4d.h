@@ -178,8 +178,10 @@ int fd_mask_words(int light_res);
  * "rect" (8 floats per receiver), "key", "sorted_id", "bin_start", "bin_end",
  * "bin_maxz", "bin_minrange", "tri_aabb", "tri_valid", "tri_data",
  * "tri_count", "tri_range", "pair_tri", "bin_tri_start", "bin_tri_end",
- * "light" (light block floats), "state" (raw state).  Synchronises the
- * device.  *bytes receives the size; copies min(size, max_bytes). */
+ * "light" (light block floats), "state" (raw state).  "sorted_id" lists each
+ * bin's receivers by ascending id (the reference's stable order; the kernels
+ * themselves walk a bin in 8x8-pixel tile order).  Synchronises the device.
+ * *bytes receives the size; copies min(size, max_bytes). */
 int fd_debug_read(fd_ctx* ctx, const char* name, void* host_dst, size_t max_bytes, size_t* bytes);
 
 /* Verification only: evaluates the light-row mask primitive of the

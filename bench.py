@@ -69,6 +69,14 @@ def gpu_plain(pkg, frame, steps, warmup, quick=False):
     ms_per_step = (t1 - t0) * 1e3 / steps
     rc, st = ctx.get_stats(stream)
     if quick:
+        # a device-to-device copy of known size: calibrates HBM byte counters
+        # (FETCH_SIZE / WRITE_SIZE) in profiler runs
+        nb = 256 << 20
+        src, dst = fd.DeviceBuffer(nb), fd.DeviceBuffer(nb)
+        fd.lib().fd_memcpy_dtod(fd.ctypes.c_void_p(dst.ptr), fd.ctypes.c_void_p(src.ptr), nb, fd.ctypes.c_void_p(stream))
+        strm.sync()
+        src.close()
+        dst.close()
         return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=[ms_per_step], stage=[0.0] * 5,
                     ops=[0] * 6, stats=st.as_dict(), shadow=None, masks=None)
     # latency: one frame at a time, host wall clock around fd_launch

@@ -765,15 +765,16 @@ __device__ __forceinline__ bool rcp_ok(float x) {
 }
 
 // kk for one (D, divisor) in the division branch; sets slow when the exact
-// quotient is needed
-__device__ __forceinline__ int k_fast(float D, float r, bool ok, float Rf, int R, bool& slow) {
+// quotient is needed.  With |divisor| in [2^-100, 2^100] the reciprocal
+// quotient q is within 2e-5 of the exact one for q <= R + 1, and the exact
+// quotient is >= 0 in this branch, so q > -2e-5: anything within 2^-12 of an
+// integer (0 included) goes to the exact path.  q > R + 1 means all columns.
+__device__ __forceinline__ int k_fast(float D, float r, bool ok, float Rf, bool& slow) {
     const float q = (-D) * r;
     const float dist = fabsf(q - rintf(q));
-    const bool big = ok && q >= Rf + 1.0f;
-    const bool good = ok && q >= 0.0f && dist > 2.44140625e-4f;
-    slow = !(big || good);
-    const float c = fmaxf(fminf(ceilf(q), Rf), 0.0f);   // finite, in [0, R]
-    return big ? R : (int)c;
+    const bool fine = ok && (dist > 2.44140625e-4f || q > Rf + 1.0f);
+    slow = !fine;
+    return (int)__builtin_amdgcn_fmed3f(ceilf(q), 0.0f, Rf);   // NaN -> 0, then in [0, R]
 }
 
 template <typename T>
@@ -782,9 +783,10 @@ __device__ __forceinline__ T low_mask(int kk) {
 }
 
 // AND of the three edge masks at plane offsets D[k] (row masks: divisor B,
-// row-range masks: divisor A).  Each edge mask is all, none, a prefix
-// [0, kk) (f1 && !f2) or a suffix [kk, R) (!f1 && f2), so their AND is the
-// column interval [max of suffix starts, min of prefix ends).
+// row-range masks: divisor A).  Each edge mask is all (f1 && f2), none
+// (!f1 && !f2), a prefix [0, kk) (f1 && !f2) or a suffix [kk, R) (!f1 && f2);
+// with v = (f1 != f2) ? kk : R the AND is the interval
+// [max over !f1 edges of v, min over f1 edges of v).
 template <typename T, bool ROWS>
 __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3], float Rf, int R) {
     bool f1[3], f2[3], slow[3];
@@ -794,7 +796,7 @@ __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3]
     for (int k = 0; k < 3; ++k) {
         f1[k] = D[k] >= 0.0f;
         f2[k] = ((ROWS ? e[k].rowN : e[k].colN) + D[k]) >= 0.0f;
-        kk[k] = k_fast(D[k], ROWS ? e[k].rA : e[k].rB, ROWS ? e[k].okA : e[k].okB, Rf, R, slow[k]);
+        kk[k] = k_fast(D[k], ROWS ? e[k].rA : e[k].rB, ROWS ? e[k].okA : e[k].okB, Rf, slow[k]);
         slow[k] = slow[k] && (f1[k] != f2[k]);
         anySlow |= slow[k];
     }
@@ -803,12 +805,11 @@ __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3]
         for (int k = 0; k < 3; ++k)
             if (slow[k]) kk[k] = k_exact(D[k], ROWS ? e[k].A : e[k].B, R);
     }
-    int hi = R, lo = 0;
+    int v[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        hi = min(hi, f2[k] ? R : (f1[k] ? kk[k] : 0));
-        lo = max(lo, (!f1[k] && f2[k]) ? kk[k] : 0);
-    }
+    for (int k = 0; k < 3; ++k) v[k] = (f1[k] != f2[k]) ? kk[k] : R;
+    const int hi = min(min(f1[0] ? v[0] : R, f1[1] ? v[1] : R), f1[2] ? v[2] : R);
+    const int lo = max(max(f1[0] ? 0 : v[0], f1[1] ? 0 : v[1]), f1[2] ? 0 : v[2]);
     return low_mask<T>(hi) & ~low_mask<T>(lo);
 }
 

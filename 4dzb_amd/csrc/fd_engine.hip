@@ -119,9 +119,8 @@ struct fd_ctx {
     fd_area_light lightDesc{};
     FdLight light{};
     unsigned long long pairCap = 64ull << 20;
-    // workspace
-    DevBuf st, rec, rxKey, histR, sortedId, sortedRec, binStat, triAABB, tri, triCount, triRange, histT, sortedTri,
-        tables, scanPart, scratchMasks;
+    // workspace (one arena, laid out per frame by layout())
+    DevBuf arena;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     FdArgs last{};
     bool haveLast = false;
@@ -129,7 +128,6 @@ struct fd_ctx {
     bool profile = false;
     bool countOps = false;
     hipEvent_t stageEv[FD_NSTAGES + 1] = {};
-    DevBuf ops;
 };
 
 namespace {
@@ -142,66 +140,88 @@ int hip_status(hipError_t e) {
     return FD_ERR_HIP;
 }
 
+// Frame arguments that do not depend on workspace placement.
+void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm, int W, int H, const FdLight& L,
+               float eps, int cull, int gridLog2, float* shadow, uint32_t* masks, unsigned long long pairCap) {
+    std::memset(&a, 0, sizeof(a));
+    a.L = L;
+    a.eps = eps;
+    a.cull = cull;
+    a.gridLog2 = gridLog2;
+    a.verts = md.verts;
+    a.vstride = md.vstride;
+    a.nverts = md.nverts;
+    a.idx = md.idx;
+    a.ntris = md.ntris;
+    a.pos = pos;
+    a.nrm = nrm;
+    a.W = W;
+    a.H = H;
+    a.WH = W * H;
+    a.shadow = shadow;
+    a.masks = masks;
+    a.pairCap = pairCap;
+    const int nb = 1 << (2 * gridLog2);
+    a.npos = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+    a.nblkR = cdiv(a.npos, FD_RXB);
+    a.nblkT = cdiv(std::max(md.ntris, 1), FD_TRB);
+    a.nR = (long)(nb + 1) * a.nblkR;
+    a.nT = (long)nb * a.nblkT;
+    a.nbR = fdk::reduce_blocks(a.WH);
+    a.nbV = fdk::reduce_blocks(a.nverts);
+}
+
+// Place the frame's workspace at `base` (nullptr: size only); returns bytes.
+size_t layout(FdArgs& a, char* base, bool countOps) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        off = (off + 255) & ~(size_t)255;
+        void* p = base ? base + off : nullptr;
+        off += bytes ? bytes : 16;
+        return p;
+    };
+    const size_t WH = (size_t)a.WH, T = (size_t)std::max(a.ntris, 1);
+    const int nb = 1 << (2 * a.gridLog2);
+    a.st = (FdState*)take(sizeof(FdState));
+    a.rec = (float4*)take(WH * sizeof(float4));
+    a.rxKey = (uint16_t*)take(WH * sizeof(uint16_t));
+    a.histR = (uint32_t*)take(a.nR * sizeof(uint32_t));
+    a.sortedId = (int*)take(WH * sizeof(int));
+    a.sortedRec = (FdRxRec*)take(WH * sizeof(FdRxRec));
+    a.binStat = (uint32_t*)take((size_t)nb * 5 * sizeof(uint32_t));
+    a.triAABB = (float4*)take(2 * T * sizeof(float4));
+    a.tri = (FdTri*)take(T * sizeof(FdTri));
+    a.triCount = (int*)take(T * sizeof(int));
+    a.triRange = (uint2*)take(T * sizeof(uint2));
+    a.histT = (uint32_t*)take(a.nT * sizeof(uint32_t));
+    a.sortedTri = (int*)take((size_t)a.pairCap * sizeof(int));
+    int* tb = (int*)take((size_t)(5 * (nb + 2)) * sizeof(int));
+    a.rxStart = tb;
+    a.rxEnd = tb ? tb + (nb + 2) : nullptr;
+    a.triStart = tb ? tb + 2 * (nb + 2) : nullptr;
+    a.triEnd = tb ? tb + 3 * (nb + 2) : nullptr;
+    a.chunkStart = tb ? tb + 4 * (nb + 2) : nullptr;
+    const long np = std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)) + 1;
+    a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));
+    a.opCount = countOps ? (unsigned long long*)take(FD_NOPS * sizeof(unsigned long long)) : nullptr;
+    if (!a.masks) a.masks = (uint32_t*)take(WH * a.L.res * a.L.wpr * sizeof(uint32_t));
+    return off;
+}
+
 // Size the workspace for the current view / scene / grid and fill FdArgs.
 int prepare(fd_ctx* c, FdArgs& a) {
     if (c->models.empty() || !c->pos || !c->nrm || !c->shadow || c->W <= 0 || c->H <= 0 || !c->lightSet)
         return FD_ERR_STATE;
     if (c->lightDesc.res_w != c->lightDesc.res_h || c->lightDesc.res_w < 1 || c->lightDesc.res_w > 64)
         return FD_ERR_UNSUPPORTED_LIGHT;
-    const Model& md = c->models[0];
-    std::memset(&a, 0, sizeof(a));
-    a.L = c->light;
-    a.eps = c->eps;
-    a.cull = c->cull;
-    a.gridLog2 = c->gridLog2;
-    a.verts = md.verts;
-    a.vstride = md.vstride;
-    a.nverts = md.nverts;
-    a.idx = md.idx;
-    a.ntris = md.ntris;
-    a.pos = c->pos;
-    a.nrm = c->nrm;
-    a.W = c->W;
-    a.H = c->H;
-    a.WH = c->W * c->H;
-    a.shadow = c->shadow;
-    const int nb = 1 << (2 * c->gridLog2);
-    const long WH = a.WH;
-    const int R = c->light.res, wpr = c->light.wpr;
-    a.npos = ((c->W + 7) / 8) * ((c->H + 7) / 8) * 64;
-    a.nblkR = cdiv(a.npos, FD_RXB);
-    a.nblkT = cdiv(std::max(md.ntris, 1), FD_TRB);
-    hipError_t e = hipSuccess;
-    auto need = [&](DevBuf& b, size_t bytes) { if (e == hipSuccess) e = b.ensure(bytes); return b.p; };
-    a.st = (FdState*)need(c->st, sizeof(FdState));
-    a.rec = (float4*)need(c->rec, WH * sizeof(float4));
-    a.rxKey = (uint16_t*)need(c->rxKey, WH * sizeof(uint16_t));
-    const long nR = (long)(nb + 1) * a.nblkR, nT = (long)nb * a.nblkT;
-    a.histR = (uint32_t*)need(c->histR, nR * sizeof(uint32_t));
-    a.sortedId = (int*)need(c->sortedId, WH * sizeof(int));
-    a.sortedRec = (FdRxRec*)need(c->sortedRec, WH * sizeof(FdRxRec));
-    a.binStat = (uint32_t*)need(c->binStat, (size_t)nb * 5 * sizeof(uint32_t));
-    const size_t T = (size_t)std::max(md.ntris, 1);
-    a.triAABB = (float4*)need(c->triAABB, 2 * T * sizeof(float4));
-    a.tri = (FdTri*)need(c->tri, T * sizeof(FdTri));
-    a.triCount = (int*)need(c->triCount, T * sizeof(int));
-    a.triRange = (uint2*)need(c->triRange, T * sizeof(uint2));
-    a.histT = (uint32_t*)need(c->histT, nT * sizeof(uint32_t));
-    a.pairCap = c->pairCap;
-    a.sortedTri = (int*)need(c->sortedTri, (size_t)c->pairCap * sizeof(int));
-    int* tb = (int*)need(c->tables, (size_t)(5 * (nb + 2)) * sizeof(int));
-    a.rxStart = tb;
-    a.rxEnd = tb + (nb + 2);
-    a.triStart = tb + 2 * (nb + 2);
-    a.triEnd = tb + 3 * (nb + 2);
-    a.chunkStart = tb + 4 * (nb + 2);
-    const long np = std::max(fdk::scan_partials_needed(nR), fdk::scan_partials_needed(nT)) + 1;
-    a.scanPart = (uint64_t*)need(c->scanPart, np * sizeof(uint64_t));
-    a.opCount = nullptr;
-    if (c->countOps) a.opCount = (unsigned long long*)need(c->ops, FD_NOPS * sizeof(unsigned long long));
-    if (c->masks) a.masks = c->masks;
-    else a.masks = (uint32_t*)need(c->scratchMasks, (size_t)WH * R * wpr * sizeof(uint32_t));
-    return hip_status(e);
+    fill_args(a, c->models[0], c->pos, c->nrm, c->W, c->H, c->light, c->eps, c->cull, c->gridLog2, c->shadow,
+              c->masks, c->pairCap);
+    FdArgs probe = a;
+    const size_t bytes = layout(probe, nullptr, c->countOps);
+    hipError_t e = c->arena.ensure(bytes);
+    if (e != hipSuccess) return hip_status(e);
+    layout(a, (char*)c->arena.p, c->countOps);
+    return FD_OK;
 }
 
 void fill_stats(const FdState& s, fd_stats* o) {
@@ -261,11 +281,7 @@ int fd_init(fd_ctx** out) {
 int fd_destroy(fd_ctx* c) {
     if (!c) return FD_ERR_ARG;
     (void)hipDeviceSynchronize();
-    DevBuf* bufs[] = {&c->st, &c->rec, &c->rxKey, &c->histR, &c->sortedId, &c->sortedRec, &c->binStat,
-                      &c->triAABB, &c->tri, &c->triCount, &c->triRange, &c->histT, &c->sortedTri,
-                      &c->tables, &c->scanPart, &c->scratchMasks};
-    for (DevBuf* b : bufs) b->release();
-    c->ops.release();
+    c->arena.release();
     for (hipEvent_t& ev : c->stageEv)
         if (ev) (void)hipEventDestroy(ev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -367,7 +383,7 @@ int fd_launch_async(fd_ctx* c, void* stream) {
     if (rc != FD_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSuccess;
-    if (a.opCount) e = hipMemsetAsync(a.opCount, 0, FD_NOPS * sizeof(unsigned long long), st);
+    if (a.opCount) e = hipMemsetAsync(a.opCount, 0, FD_NOPS * sizeof(unsigned long long), st);  // profiling only
     if (e == hipSuccess) e = fdk::enqueue_frame(a, st, c->profile ? c->stageEv : nullptr);
     c->last = a;
     c->haveLast = true;
@@ -505,6 +521,168 @@ const char* fd_status_string(int code) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched form: many frames (views / lights of uneven sizes) over the
+// context's scene, one launch per stage for all of them, optionally captured
+// into a HIP graph.
+// ---------------------------------------------------------------------------
+struct fd_batch {
+    fd_ctx* ctx = nullptr;
+    int n = 0;
+    std::vector<FdArgs> args;
+    DevBuf arena, tables;
+    FdBatchPlan plan{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch** out) {
+    if (!c || !in || n <= 0 || !out) return FD_ERR_ARG;
+    if (c->models.empty()) return FD_ERR_STATE;
+    const Model& md = c->models[0];
+    fd_batch* b = new fd_batch();
+    b->ctx = c;
+    b->n = n;
+    b->args.resize(n);
+    std::vector<size_t> off(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        const fd_instance& q = in[i];
+        if (!q.d_position4 || !q.d_normal4 || !q.d_shadow || q.width <= 0 || q.height <= 0 ||
+            (long)q.width * q.height > (1l << 28)) { delete b; return FD_ERR_ARG; }
+        if (q.light.res_w != q.light.res_h || q.light.res_w < 1 || q.light.res_w > 64) {
+            delete b;
+            return FD_ERR_UNSUPPORTED_LIGHT;
+        }
+        int g = c->gridLog2;
+        if (q.bins_per_side) {
+            int l = 0;
+            while ((1 << l) < q.bins_per_side) ++l;
+            if ((1 << l) != q.bins_per_side || l < 3 || l > 6) { delete b; return FD_ERR_ARG; }
+            g = l;
+        }
+        FdLight L;
+        light_setup(q.light, L);
+        const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
+        fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
+                  c->eps, c->cull, g, q.d_shadow, q.d_masks, cap);
+        FdArgs probe = b->args[i];
+        off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
+    }
+    hipError_t e = b->arena.ensure(off[n]);
+    if (e != hipSuccess) { delete b; return hip_status(e); }
+    for (int i = 0; i < n; ++i) layout(b->args[i], (char*)b->arena.p + off[i], false);
+    // block maps
+    FdBatchPlan& P = b->plan;
+    P.n = n;
+    P.maxNb = 0;
+    P.anyTris = md.ntris > 0;
+    std::vector<int> occIdx[2];
+    for (int i = 0; i < n; ++i) {
+        P.maxNb = std::max(P.maxNb, 1 << (2 * b->args[i].gridLog2));
+        occIdx[b->args[i].L.wpr - 1].push_back(i);
+    }
+    for (int cl = 0; cl < 2; ++cl) {  // largest work first: shorter tail
+        auto cost = [&](int i) { return (double)b->args[i].WH * b->args[i].L.res * b->args[i].L.res; };
+        std::stable_sort(occIdx[cl].begin(), occIdx[cl].end(), [&](int x, int y) { return cost(x) > cost(y); });
+        P.nocc[cl] = (int)occIdx[cl].size();
+        P.maxRes[cl] = 1;
+        for (int i : occIdx[cl]) P.maxRes[cl] = std::max(P.maxRes[cl], b->args[i].L.res);
+    }
+    std::vector<int> maps;
+    int mapOff[FD_NMAPS];
+    for (int m = 0; m < FD_NMAPS; ++m) {
+        mapOff[m] = (int)maps.size();
+        const bool occ = (m == FD_MAP_OCC32 || m == FD_MAP_OCC64);
+        const std::vector<int>* order = occ ? &occIdx[m == FD_MAP_OCC64 ? 1 : 0] : nullptr;
+        const int cnt = occ ? (int)order->size() : n;
+        int acc = 0;
+        maps.push_back(0);
+        for (int k = 0; k < cnt; ++k) {
+            const FdArgs& a = b->args[occ ? (*order)[k] : k];
+            const int nb = 1 << (2 * a.gridLog2);
+            int blocks = 1;
+            switch (m) {
+                case FD_MAP_ONE: blocks = 1; break;
+                case FD_MAP_BINS: blocks = cdiv(nb, 256); break;
+                case FD_MAP_RED: blocks = a.nbR + a.nbV; break;
+                case FD_MAP_RX: blocks = a.nblkR; break;
+                case FD_MAP_SCANR: blocks = fdk::scan_chunks(a.nR); break;
+                case FD_MAP_TRI: blocks = a.nblkT; break;
+                case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
+                default: blocks = cdiv(a.WH, 64) + nb + 1; break;
+            }
+            acc += blocks;
+            maps.push_back(acc);
+        }
+        P.total[m] = acc;
+    }
+    std::vector<FdArgs> occTab[2];
+    for (int cl = 0; cl < 2; ++cl)
+        for (int i : occIdx[cl]) occTab[cl].push_back(b->args[i]);
+    const size_t argBytes = sizeof(FdArgs) * (size_t)(n + P.nocc[0] + P.nocc[1]);
+    e = b->tables.ensure(argBytes + maps.size() * sizeof(int));
+    if (e == hipSuccess) {
+        FdArgs* dArgs = (FdArgs*)b->tables.p;
+        e = hipMemcpy(dArgs, b->args.data(), n * sizeof(FdArgs), hipMemcpyHostToDevice);
+        if (e == hipSuccess && P.nocc[0])
+            e = hipMemcpy(dArgs + n, occTab[0].data(), P.nocc[0] * sizeof(FdArgs), hipMemcpyHostToDevice);
+        if (e == hipSuccess && P.nocc[1])
+            e = hipMemcpy(dArgs + n + P.nocc[0], occTab[1].data(), P.nocc[1] * sizeof(FdArgs), hipMemcpyHostToDevice);
+        int* dMaps = (int*)((char*)b->tables.p + argBytes);
+        if (e == hipSuccess) e = hipMemcpy(dMaps, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice);
+        P.inst = dArgs;
+        P.occ[0] = dArgs + n;
+        P.occ[1] = dArgs + n + P.nocc[0];
+        for (int m = 0; m < FD_NMAPS; ++m) P.map[m] = dMaps + mapOff[m];
+    }
+    if (e != hipSuccess) { delete b; return hip_status(e); }
+    *out = b;
+    return FD_OK;
+}
+
+extern "C" int fd_batch_capture(fd_batch* b, void* stream) {
+    if (!b) return FD_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    if (!st) return FD_ERR_ARG;  // capture needs a created stream
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_status(e);
+    hipError_t e2 = fdk::enqueue_batch(b->plan, st);
+    e = hipStreamEndCapture(st, &b->graph);
+    if (e2 != hipSuccess) return hip_status(e2);
+    if (e != hipSuccess) return hip_status(e);
+    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    return hip_status(e);
+}
+
+extern "C" int fd_batch_launch(fd_batch* b, void* stream) {
+    if (!b) return FD_ERR_ARG;
+    if (b->exec) return hip_status(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return hip_status(fdk::enqueue_batch(b->plan, (hipStream_t)stream));
+}
+
+extern "C" int fd_batch_get_stats(fd_batch* b, int i, fd_stats* out) {
+    if (!b || i < 0 || i >= b->n || !out) return FD_ERR_ARG;
+    hipError_t e = hipDeviceSynchronize();
+    FdState s;
+    if (e == hipSuccess) e = hipMemcpy(&s, b->args[i].st, sizeof(s), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_status(e);
+    fill_stats(s, out);
+    return s.status == FD_EXIT_CAPACITY ? FD_ERR_CAPACITY : FD_OK;
+}
+
+extern "C" int fd_batch_destroy(fd_batch* b) {
+    if (!b) return FD_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->arena.release();
+    b->tables.release();
+    delete b;
+    return FD_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Stage read-back (verification only; synchronises)

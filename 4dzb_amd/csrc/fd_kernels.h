@@ -33,8 +33,10 @@ struct FdArgs {
     float4* rec;          // WH: (x', y', z, tag)
     uint16_t* rxKey;      // WH
     uint32_t* histR;      // (nb + 1) * nblkR, scanned in place
-    int nblkR;
-    int nblkT;
+    int nblkR;            // blocks of the receiver key / scatter passes
+    int nblkT;            // blocks of the triangle count / scatter passes
+    int nbR, nbV;         // k_reduce blocks over receivers / vertices
+    long nR, nT;          // lengths of the receiver / pair histograms
     int* sortedId;        // WH
     FdRxRec* sortedRec;   // WH
     uint32_t* binStat;    // nb * 5 (order-preserving encodings)
@@ -61,7 +63,25 @@ struct FdArgs {
 // Stage boundaries recorded when profiling (events[0..FD_NSTAGES]).
 #define FD_NSTAGES 5   // light plane, receiver binning, triangle pairing, tables, occlusion
 
+// Block maps of a batched launch (prefix sums over instances, n + 1 each).
+enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,
+             FD_MAP_OCC64, FD_NMAPS };
+
+struct FdBatchPlan {
+    const FdArgs* inst;       // device: instance table (n)
+    const FdArgs* occ[2];     // device: instances of the u32 / u64 occlusion classes, largest first
+    int n, nocc[2];
+    const int* map[FD_NMAPS]; // device prefix arrays
+    int total[FD_NMAPS];      // blocks per stage launch
+    int maxNb;                // largest bin count (dynamic LDS sizes)
+    int maxRes[2];            // largest light grid per occlusion class
+    bool anyTris;
+};
+
 namespace fdk {
 hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* stageEvents = nullptr);
+hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st);
 long scan_partials_needed(long n);
+int reduce_blocks(long n);
+int scan_chunks(long n);
 }  // namespace fdk

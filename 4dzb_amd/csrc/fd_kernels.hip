@@ -32,8 +32,8 @@ constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;
 // k_init: reset the frame state (a kernel, not a memset, so that replays of
 // a captured graph start from the same state).
 // --------------------------------------------------------------------------
-__global__ void k_init(FdArgs a) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void init_body(const FdArgs& a, const int blk) {
+    if (threadIdx.x != 0 || blk != 0) return;
     FdState* s = a.st;
     const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
     s->rxZmaxAll = encNegMax;
@@ -56,8 +56,8 @@ __global__ void k_init(FdArgs a) {
 }
 
 // Bin statistics reset (maxZ, minRange) for the bins of this frame.
-__global__ void k_init_bins(FdArgs a) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void init_bins_body(const FdArgs& a, const int blk) {
+    int b = blk * blockDim.x + threadIdx.x;
     const int nb = 1 << (2 * a.gridLog2);
     if (b >= nb) return;
     const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
@@ -97,13 +97,14 @@ __device__ void block_reduce_atomic(uint32_t (&v)[NV], const int (&op)[NV], uint
 // known: rounding of (a - S) is monotone in a, so both give the same float.
 // Vertices: Step2_2_modelRectangleAABB.inl:1-99.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(RED_BLOCK) void k_reduce(FdArgs a, int nbR) {
+__device__ __forceinline__ void reduce_body(const FdArgs& a, const int blk) {
+    const int nbR = a.nbR;
     const float* m = a.L.m;
     const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
-    if ((int)blockIdx.x < nbR) {
+    if ((int)blk < nbR) {
         uint32_t v[7] = {encNegMax, encMax, encMax, encMax, encNegMax, encNegMax, encNegMax};
         int cnt = 0;
-        const int base = blockIdx.x * RED_BLOCK * RED_ITEMS + threadIdx.x;
+        const int base = blk * RED_BLOCK * RED_ITEMS + threadIdx.x;
 #pragma unroll
         for (int it = 0; it < RED_ITEMS; ++it) {
             const int i = base + it * RED_BLOCK;
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(RED_BLOCK) void k_reduce(FdArgs a, int nbR) {
         if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s->validRect, cnt);
     } else {
         uint32_t v[7] = {encNegMax, encMax, encMax, encMax, encNegMax, encNegMax, encNegMax};
-        const int base = (blockIdx.x - nbR) * RED_BLOCK * RED_ITEMS + threadIdx.x;
+        const int base = (blk - nbR) * RED_BLOCK * RED_ITEMS + threadIdx.x;
 #pragma unroll
         for (int it = 0; it < RED_ITEMS; ++it) {
             const int vi = base + it * RED_BLOCK;
@@ -175,8 +176,8 @@ __global__ __launch_bounds__(RED_BLOCK) void k_reduce(FdArgs a, int nbR) {
 // k_finalize: the host part of lightPlaneParamCal
 // (Step2_3_defineFDbuffer.inl:5-111) on the device, with its early exits.
 // --------------------------------------------------------------------------
-__global__ void k_finalize(FdArgs a) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void finalize_body(const FdArgs& a, const int blk) {
+    if (threadIdx.x != 0 || blk != 0) return;
     FdState* s = a.st;
     // sampleMaxZcal / triangleMaxZcal: thrust::reduce(max) seeded with -1
     const float zs = fmax_(-1.0f, fdec(s->rxZmaxAll));
@@ -257,7 +258,7 @@ __device__ __forceinline__ int tile_id(int p, int W, int H) {
     return (x < W && y < H) ? y * W + x : -1;
 }
 
-__global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
+__device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t hist[];  // nb + 1
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
     for (int k = threadIdx.x; k <= nb; k += blockDim.x) hist[k] = 0;
     __syncthreads();
     int nvalid = 0, noor = 0;
-    const int base = blockIdx.x * FD_RXB;
+    const int base = blk * FD_RXB;
     for (int it = 0; it < FD_RXB / 256; ++it) {
         const int p = base + it * 256 + threadIdx.x;
         const int i = p < a.npos ? tile_id(p, a.W, a.H) : -1;
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) {
         if (noor) atomicAdd(&a.st->keyOutOfRange, noor);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k <= nb; k += blockDim.x) a.histR[(size_t)k * a.nblkR + blockIdx.x] = hist[k];
+    for (int k = threadIdx.x; k <= nb; k += blockDim.x) a.histR[(size_t)k * a.nblkR + blk] = hist[k];
 }
 
 // --------------------------------------------------------------------------
@@ -349,21 +350,21 @@ __device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* to
     return off + x - v;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ d, long n,
-                                                            uint64_t* __restrict__ part, const int* status) {
+__device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ d, long n, uint64_t* __restrict__ part,
+                                                 const int* status, const int blk) {
     if (status && *status != FD_EXIT_OK) return;
-    const long base = (long)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    const long base = (long)blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
     uint64_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k)
         if (base + k < n) sum += d[base + k];
     uint64_t tot;
     block_excl_scan_u64(sum, &tot);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+    if (threadIdx.x == 0) part[blk] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_partials(uint64_t* __restrict__ part, int np,
-                                                              unsigned long long* total, const int* status) {
+__device__ __forceinline__ void scan_partials_body(uint64_t* __restrict__ part, int np, unsigned long long* total,
+                                                   const int* status) {
     if (status && *status != FD_EXIT_OK) return;
     uint64_t carry = 0;
     for (int b0 = 0; b0 < np; b0 += SCAN_BLOCK) {
@@ -377,10 +378,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_partials(uint64_t* __restri
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict__ d, long n,
-                                                           const uint64_t* __restrict__ part, const int* status) {
+__device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ d, long n, const uint64_t* __restrict__ part,
+                                                const int* status, const int blk) {
     if (status && *status != FD_EXIT_OK) return;
-    const long base = (long)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    const long base = (long)blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint64_t sum = 0;
 #pragma unroll
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict_
         sum += v[k];
     }
     uint64_t tot;
-    uint64_t off = block_excl_scan_u64(sum, &tot) + part[blockIdx.x];
+    uint64_t off = block_excl_scan_u64(sum, &tot) + part[blk];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         if (base + k < n) d[base + k] = (uint32_t)off;
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict_
 // sort_by_key (Step3_bindSampleToBin.inl:125-127).  Binned receivers also
 // get their 32-byte occlusion record written in sorted order.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) {
+__device__ __forceinline__ void rx_scatter_body(const FdArgs& a, const int blk) {
     extern __shared__ int cnt[];  // nb + 1, -1 = not loaded yet
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) {
     while ((1 << kbits) <= nb) ++kbits;
     for (int k = threadIdx.x; k <= nb; k += 64) cnt[k] = -1;
     __syncthreads();
-    const int base = blockIdx.x * FD_RXB;
+    const int base = blk * FD_RXB;
     for (int it = 0; it < FD_RXB / 64; ++it) {
         const int pp = base + it * 64 + lane_id();
         const int i = pp < a.npos ? tile_id(pp, a.W, a.H) : -1;
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) {
             const int rank = mbcnt(eq);
             const int gcount = __popcll(eq);
             int b0 = cnt[key];
-            if (b0 < 0) b0 = (int)a.histR[(size_t)key * a.nblkR + blockIdx.x];
+            if (b0 < 0) b0 = (int)a.histR[(size_t)key * a.nblkR + blk];
             const int p = b0 + rank;
             if (rank == gcount - 1) cnt[key] = b0 + gcount;
             if ((unsigned)p < (unsigned)a.WH) a.sortedId[p] = i;
@@ -489,14 +490,14 @@ __device__ __forceinline__ TriClamp tri_clamp(const float4 mn, const float4 mx, 
     return c;
 }
 
-__global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) {
+__device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t thist[];  // nb
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) thist[k] = 0;
     __syncthreads();
-    const int t = blockIdx.x * FD_TRB + threadIdx.x;
+    const int t = blk * FD_TRB + threadIdx.x;
     const bool act = t < a.ntris;
     const float* m = a.L.m;
     float4 mn = make_float4(FD_F32_MAX, FD_F32_MAX, FD_F32_MAX, 1.0f);
@@ -597,12 +598,12 @@ __global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) {
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blockIdx.x] = thist[k];
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blk] = thist[k];
 }
 
 // Stable pair scatter (writeTriBinPair_kernel + sort_by_key,
 // Step4_RasterizeTriangleToBin.inl:161-223): per bin, triangles ascending.
-__global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) {
+__device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk) {
     extern __shared__ int tcnt[];  // nb, -1 = not loaded yet
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) {
     __syncthreads();
     const unsigned long long cap = a.pairCap;
     for (int it = 0; it < FD_TRB / 64; ++it) {
-        const int t = blockIdx.x * FD_TRB + it * 64 + lane_id();
+        const int t = blk * FD_TRB + it * 64 + lane_id();
         const bool act = t < a.ntris;
         float4 mn = make_float4(0, 0, 0, 0), mx = mn;
         uint2 rg = make_uint2(6000u | (6000u << 16), 0u);
@@ -632,7 +633,7 @@ __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) {
                 const uint64_t bal = __ballot(pass);
                 if (bal) {
                     int b0 = tcnt[bp];
-                    if (b0 < 0) b0 = (int)a.histT[(size_t)bp * a.nblkT + blockIdx.x];
+                    if (b0 < 0) b0 = (int)a.histT[(size_t)bp * a.nblkT + blk];
                     if (pass) {
                         const unsigned long long p = (unsigned long long)(uint32_t)b0 + (unsigned)mbcnt(bal);
                         if (p < cap) a.sortedTri[p] = t;
@@ -650,7 +651,7 @@ __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) {
 // work list: one work item per 64 sorted receivers of a bin (the invalid key
 // included, so every receiver's output is written exactly once).
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_tables(FdArgs a) {
+__device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
     __shared__ int wsum[16];
     FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
@@ -699,8 +700,8 @@ __global__ __launch_bounds__(1024) void k_tables(FdArgs a) {
 // Exit bookkeeping after the scans (setupTriangleVertex returns 1 when no
 // triangle reached the light plane, Step0_triangleSetup.inl:224-225;
 // FDLaunch skips the rest when there are no pairs, FD/api.cpp:522).
-__global__ void k_post_pairs(FdArgs a) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) {
+    if (threadIdx.x != 0 || blk != 0) return;
     FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     if (s->validTris == 0) s->status = FD_EXIT_NO_VALID_TRI;
@@ -841,7 +842,7 @@ __device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds
 }
 
 template <typename T, bool COUNT>
-__global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
+__device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 ids
     const FdState* s = a.st;
     const int status = s->status;
@@ -850,7 +851,7 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     const int lane = lane_id();
     int* ids = (int*)(lds + wpr * R * 64);
     const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
-    const int c = blockIdx.x;
+    const int c = blk;
 
     if (status != FD_EXIT_OK) {
         // early exit of the frame: every receiver shadow 0, except back-facing
@@ -1042,6 +1043,97 @@ __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     store_masks<T>(a, lds, ids, n, R, wpr);
 }
 
+// ==========================================================================
+// Kernel entry points.  Single frame: the FdArgs block travels as the kernel
+// argument.  Batched: every stage is ONE launch over all instances; block b
+// belongs to instance i with map[i] <= b < map[i+1] (host-built prefix), and
+// the instance's FdArgs is read from the device-resident instance table.
+// ==========================================================================
+__global__ void k_init(FdArgs a) { init_body(a, blockIdx.x); }
+__global__ void k_init_bins(FdArgs a) { init_bins_body(a, blockIdx.x); }
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce(FdArgs a) { reduce_body(a, blockIdx.x); }
+__global__ void k_finalize(FdArgs a) { finalize_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_rx_key(FdArgs a) { rx_key_body(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) { rx_scatter_body(a, blockIdx.x); }
+__global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) { tri_count_body(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) { tri_scatter_body(a, blockIdx.x); }
+__global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
+__global__ void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
+template <typename T, bool COUNT>
+__global__ __launch_bounds__(64) void k_occlude(FdArgs a) { occlude_body<T, COUNT>(a, blockIdx.x); }
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ d, long n,
+                                                            uint64_t* __restrict__ part, const int* status) {
+    scan_reduce_body(d, n, part, status, blockIdx.x);
+}
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_partials(uint64_t* __restrict__ part, int np,
+                                                              unsigned long long* total, const int* status) {
+    scan_partials_body(part, np, total, status);
+}
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict__ d, long n,
+                                                           const uint64_t* __restrict__ part, const int* status) {
+    scan_apply_body(d, n, part, status, blockIdx.x);
+}
+
+// instance of block b: map[i] <= b < map[i+1] (wave-uniform binary search)
+__device__ __forceinline__ int inst_of(const int* __restrict__ map, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (map[mid] <= b) lo = mid; else hi = mid;
+    }
+    return uniform(lo);
+}
+
+#define FD_BATCHED(NAME, BOUND, BODY)                                                          \
+    __global__ __launch_bounds__(BOUND) void NAME(const FdArgs* __restrict__ inst,              \
+                                                  const int* __restrict__ map, int n) {        \
+        const int b = blockIdx.x;                                                              \
+        const int i = inst_of(map, n, b);                                                      \
+        const FdArgs a = inst[i];                                                              \
+        BODY(a, b - map[i]);                                                                   \
+    }
+FD_BATCHED(kb_init, 64, init_body)
+FD_BATCHED(kb_init_bins, 256, init_bins_body)
+FD_BATCHED(kb_reduce, RED_BLOCK, reduce_body)
+FD_BATCHED(kb_finalize, 64, finalize_body)
+FD_BATCHED(kb_rx_key, 256, rx_key_body)
+FD_BATCHED(kb_rx_scatter, 64, rx_scatter_body)
+FD_BATCHED(kb_tri_count, FD_TRB, tri_count_body)
+FD_BATCHED(kb_tri_scatter, 64, tri_scatter_body)
+FD_BATCHED(kb_tables, 1024, tables_body)
+FD_BATCHED(kb_post_pairs, 64, post_pairs_body)
+#undef FD_BATCHED
+
+template <typename T>
+__global__ __launch_bounds__(64) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
+    const int b = blockIdx.x;
+    const int i = inst_of(map, n, b);
+    const FdArgs a = inst[i];
+    occlude_body<T, false>(a, b - map[i]);
+}
+
+// which = 0: receiver key histogram, 1: pair histogram
+__global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_reduce(const FdArgs* __restrict__ inst, const int* __restrict__ map,
+                                                             int n, int which) {
+    const int b = blockIdx.x;
+    const int i = inst_of(map, n, b);
+    const FdArgs a = inst[i];
+    scan_reduce_body(which ? a.histT : a.histR, which ? a.nT : a.nR, a.scanPart, &a.st->status, b - map[i]);
+}
+__global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_partials(const FdArgs* __restrict__ inst, int which) {
+    const FdArgs a = inst[blockIdx.x];
+    const long len = which ? a.nT : a.nR;
+    scan_partials_body(a.scanPart, (int)((len + SCAN_CHUNK - 1) / SCAN_CHUNK), which ? &a.st->pairs : &a.st->rxTotal,
+                       &a.st->status);
+}
+__global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_apply(const FdArgs* __restrict__ inst, const int* __restrict__ map,
+                                                            int n, int which) {
+    const int b = blockIdx.x;
+    const int i = inst_of(map, n, b);
+    const FdArgs a = inst[i];
+    scan_apply_body(which ? a.histT : a.histR, which ? a.nT : a.nR, a.scanPart, &a.st->status, b - map[i]);
+}
+
 // Verification of the row-mask primitive on given (D, B) pairs (n = R - 1),
 // through the same fast / exact paths as the occlusion kernel.
 __global__ void k_selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out) {
@@ -1096,6 +1188,39 @@ static hipError_t scan_u32(uint32_t* d, long n, uint64_t* part, unsigned long lo
 }
 
 long scan_partials_needed(long n) { return cdiv(n, SCAN_CHUNK); }
+int reduce_blocks(long n) { return cdiv(n, RED_BLOCK * RED_ITEMS); }
+int scan_chunks(long n) { return cdiv(n, SCAN_CHUNK); }
+
+hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
+    const int n = p.n;
+    auto L = [&](int m) { return dim3((unsigned)(p.total[m] > 0 ? p.total[m] : 1)); };
+    const int nb = p.maxNb;
+    kb_init<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    kb_init_bins<<<L(FD_MAP_BINS), 256, 0, st>>>(p.inst, p.map[FD_MAP_BINS], n);
+    kb_reduce<<<L(FD_MAP_RED), RED_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_RED], n);
+    kb_finalize<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    kb_rx_key<<<L(FD_MAP_RX), 256, (nb + 1) * sizeof(uint32_t), st>>>(p.inst, p.map[FD_MAP_RX], n);
+    kb_scan_reduce<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
+    kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 0);
+    kb_scan_apply<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
+    kb_rx_scatter<<<L(FD_MAP_RX), 64, (nb + 1) * sizeof(int), st>>>(p.inst, p.map[FD_MAP_RX], n);
+    if (p.anyTris) {
+        kb_tri_count<<<L(FD_MAP_TRI), FD_TRB, nb * sizeof(uint32_t), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        kb_scan_reduce<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
+        kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 1);
+        kb_scan_apply<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
+    }
+    kb_post_pairs<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    if (p.anyTris) kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+    kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    for (int c = 0; c < 2; ++c) {
+        if (p.nocc[c] == 0) continue;
+        const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t) + 64 * sizeof(int);
+        if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
+        else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
+    }
+    return hipGetLastError();
+}
 
 hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out) {
     k_selftest_rowmask<<<cdiv(n, 256), 256>>>(D, B, n, R, out);
@@ -1113,26 +1238,25 @@ hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st) {
 
 hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     const int nb = 1 << (2 * a.gridLog2);
-    const int nbR = cdiv(a.WH, RED_BLOCK * RED_ITEMS);
-    const int nbV = cdiv(a.nverts, RED_BLOCK * RED_ITEMS);
+    const int nbR = a.nbR, nbV = a.nbV;
     FdState* s = a.st;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], st); };
     mark(0);
     k_init<<<1, 64, 0, st>>>(a);
     k_init_bins<<<cdiv(nb, 256), 256, 0, st>>>(a);
-    k_reduce<<<nbR + nbV, RED_BLOCK, 0, st>>>(a, nbR);
+    k_reduce<<<nbR + nbV, RED_BLOCK, 0, st>>>(a);
     k_finalize<<<1, 64, 0, st>>>(a);
     mark(1);
     // receivers: key + bin statistics, stable counting sort
     k_rx_key<<<a.nblkR, 256, (nb + 1) * sizeof(uint32_t), st>>>(a);
-    hipError_t e = scan_u32(a.histR, (long)(nb + 1) * a.nblkR, a.scanPart, &s->rxTotal, &s->status, st);
+    hipError_t e = scan_u32(a.histR, a.nR, a.scanPart, &s->rxTotal, &s->status, st);
     if (e != hipSuccess) return e;
     k_rx_scatter<<<a.nblkR, 64, (nb + 1) * sizeof(int), st>>>(a);
     mark(2);
     // triangles: setup + pair count, stable pair scatter
     if (a.ntris > 0) {
         k_tri_count<<<a.nblkT, FD_TRB, nb * sizeof(uint32_t), st>>>(a);
-        e = scan_u32(a.histT, (long)nb * a.nblkT, a.scanPart, &s->pairs, &s->status, st);
+        e = scan_u32(a.histT, a.nT, a.scanPart, &s->pairs, &s->status, st);
         if (e != hipSuccess) return e;
         k_post_pairs<<<1, 64, 0, st>>>(a);
         k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);

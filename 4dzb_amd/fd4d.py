@@ -51,6 +51,13 @@ class Stats(ctypes.Structure):
                 for k, _ in self._fields_}
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("d_position4", ctypes.c_void_p), ("d_normal4", ctypes.c_void_p), ("width", ctypes.c_int),
+                ("height", ctypes.c_int), ("light", AreaLight), ("d_shadow", ctypes.c_void_p),
+                ("d_masks", ctypes.c_void_p), ("bins_per_side", ctypes.c_int), ("pad", ctypes.c_int),
+                ("max_pairs", ctypes.c_ulonglong)]
+
+
 _lib = None
 
 
@@ -88,6 +95,11 @@ def lib():
             "fd_set_profiling": ([vp, ip, ip], ip),
             "fd_selftest_rowmask": ([vp, vp, ip, ip, vp], ip),
             "fd_device_count": ([ctypes.POINTER(ip)], ip),
+            "fd_batch_create": ([vp, ctypes.c_void_p, ip, ctypes.POINTER(vp)], ip),
+            "fd_batch_capture": ([vp, vp], ip),
+            "fd_batch_launch": ([vp, vp], ip),
+            "fd_batch_get_stats": ([vp, ip, ctypes.POINTER(Stats)], ip),
+            "fd_batch_destroy": ([vp], ip),
             "fd_dev_alloc": ([sz, ctypes.POINTER(vp)], ip),
             "fd_dev_free": ([vp], ip),
             "fd_memcpy_htod": ([vp, vp, sz], ip),
@@ -338,6 +350,48 @@ class FrameRunner:
         for b in (self.verts, self.idx, self.pos, self.nrm, self.shadow, self.masks):
             if b is not None:
                 b.close()
+
+
+class Batch:
+    """Batched form: many (receivers, light, outputs) instances over one
+    context's scene, one launch per stage; optional HIP-graph capture."""
+
+    def __init__(self, ctx, items):
+        """items: list of dicts with keys pos, nrm (DeviceBuffer), w, h, light
+        (LightDesc), shadow, masks (DeviceBuffer or None), bins (0 = ctx)."""
+        arr = (Instance * len(items))()
+        for k, it in enumerate(items):
+            arr[k] = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
+                              it["shadow"].ptr, it["masks"].ptr if it.get("masks") is not None else 0,
+                              it.get("bins", 0), 0, it.get("max_pairs", 0))
+        h = ctypes.c_void_p()
+        _check(lib().fd_batch_create(ctx.h, ctypes.cast(arr, ctypes.c_void_p), len(items), ctypes.byref(h)),
+               "fd_batch_create")
+        self.h = h
+        self.n = len(items)
+        self._keep = (ctx, items)
+
+    def capture(self, stream):
+        _check(lib().fd_batch_capture(self.h, ctypes.c_void_p(stream)), "fd_batch_capture")
+
+    def launch(self, stream=None):
+        _check(lib().fd_batch_launch(self.h, ctypes.c_void_p(stream or 0)), "fd_batch_launch")
+
+    def stats(self, i):
+        st = Stats()
+        rc = lib().fd_batch_get_stats(self.h, i, ctypes.byref(st))
+        return rc, st
+
+    def close(self):
+        if self.h:
+            lib().fd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # numpy mirrors of fd_types.h records (for stage read-back)

@@ -149,6 +149,32 @@ int fd_set_pair_capacity(fd_ctx* ctx, size_t max_pairs);
  * in-place 4x4 (column-major) transform of model 0's vertex positions. */
 int fd_move_model(fd_ctx* ctx, const float m[16], void* hip_stream);
 
+/* ---- Batched form (no counterpart: the reference renders one view with one
+ * light per FDLaunch).  Many frames over the context's scene -- each with its
+ * own receivers, light and outputs -- run as ONE launch per stage (instance
+ * table in device memory), optionally captured into a HIP graph.  Results
+ * are bit-identical to running each instance alone through fd_launch. */
+typedef struct fd_instance {
+    const float* d_position4;   /* receiver G-buffer (as fd_set_sample_buffers) */
+    const float* d_normal4;
+    int width, height;
+    fd_area_light light;
+    float* d_shadow;            /* width*height floats */
+    uint32_t* d_masks;          /* width*height*fd_mask_words(res) words, or NULL */
+    int bins_per_side;          /* 0 = the context's setting */
+    int pad;
+    unsigned long long max_pairs; /* pair arena of this instance; 0 = 16M */
+} fd_instance;
+typedef struct fd_batch fd_batch;
+int fd_batch_create(fd_ctx* ctx, const fd_instance* instances, int n, fd_batch** out);
+/* Build a HIP graph of the batch on `hip_stream` (a created stream); later
+ * fd_batch_launch calls replay it. */
+int fd_batch_capture(fd_batch* batch, void* hip_stream);
+/* Enqueue (or replay) the batch; no host synchronisation. */
+int fd_batch_launch(fd_batch* batch, void* hip_stream);
+int fd_batch_get_stats(fd_batch* batch, int i, fd_stats* stats);
+int fd_batch_destroy(fd_batch* batch);
+
 /* No counterpart (the reference times the whole FDLaunch with cudaEvents,
  * FD/api.cpp:405-409,574-581): per-stage device times of the next frames
  * (light plane, receiver binning, triangle pairing, tables, occlusion) and

@@ -1,0 +1,81 @@
+"""Batched form: every instance's result is the same alone (fd_launch) and
+in the mix (one launch per stage for all instances, direct and replayed from
+a HIP graph), bit for bit (MI355X only)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (w, h, light res, light centre, bins per side)
+MIX = [(96, 96, 8, (0.5, -0.3, 5.0), 0), (160, 120, 16, (-1.0, 0.7, 4.5), 0), (256, 256, 32, (0.2, 0.1, 6.0), 0),
+       (200, 150, 33, (1.5, -1.2, 7.0), 0), (128, 128, 16, (-0.5, -1.5, 5.5), 32), (64, 200, 64, (0.0, 0.0, 6.0), 8),
+       (256, 256, 32, (1.0, 1.0, 4.2), 0)]
+
+
+def _setup(fd, wl):
+    scene = wl.spheres(80, 7)
+    ctx = fd.Context()
+    ctx.set_backface_cull(1)
+    ctx.set_scenes_esp(1e-3)
+    verts = fd.DeviceBuffer.from_array(scene.verts)
+    idx = fd.DeviceBuffer.from_array(scene.idx)
+    ctx.add_model(verts, scene.nverts, 16, idx, scene.ntris)
+    items = []
+    for (w, h, R, c, bins) in MIX:
+        rx = wl.receivers_heightfield(w, h, 8.0, 8.0 * h / w)
+        items.append(dict(w=w, h=h, bins=bins, rx=rx, light=wl.LightDesc(1.0, 1.0, c, res_w=R, res_h=R),
+                          pos=fd.DeviceBuffer.from_array(rx.pos), nrm=fd.DeviceBuffer.from_array(rx.nrm),
+                          shadow=fd.DeviceBuffer(w * h * 4), masks=fd.DeviceBuffer(w * h * fd.mask_words(R) * 4)))
+    return scene, ctx, items, (verts, idx)
+
+
+def _alone(fd, wl, scene, it):
+    fr = wl.Frame("alone", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1,
+                  grid_log2=(it["bins"].bit_length() - 1) if it["bins"] else 4)
+    run = fd.FrameRunner(fr)
+    rc, st = run.run()
+    assert rc == 0
+    out = run.outputs()
+    run.close()
+    return out, st
+
+
+def test_batch_equals_alone(fd, wl):
+    scene, ctx, items, keep = _setup(fd, wl)
+    alone = [_alone(fd, wl, scene, it) for it in items]
+    stream = fd.Stream()
+    b = fd.Batch(ctx, items)
+    for mode in ["direct", "graph", "graph-replay"]:
+        for it in items:   # poison the outputs between runs
+            it["shadow"].upload(np.full(it["w"] * it["h"], -7.0, np.float32))
+        if mode == "graph":
+            b.capture(stream.handle)
+        b.launch(stream.handle)
+        stream.sync()
+        for k, it in enumerate(items):
+            (sh0, mk0), st0 = alone[k]
+            sh = it["shadow"].download(np.float32)
+            mk = it["masks"].download(np.uint32)
+            rc, st = b.stats(k)
+            assert rc == 0 and st.exit_code == st0.exit_code and st.pairs == st0.pairs, (mode, k)
+            assert np.array_equal(sh.view(np.uint32), sh0.view(np.uint32)), (mode, k)
+            assert np.array_equal(mk, mk0), (mode, k)
+    b.close()
+    stream.close()
+    ctx.close()
+
+
+def test_batch_instance_against_oracle(fd, wl, oracle):
+    scene, ctx, items, keep = _setup(fd, wl)
+    b = fd.Batch(ctx, items)
+    b.launch()
+    fd.lib().fd_device_sync()
+    for k in (0, 3, 5):
+        it = items[k]
+        fr = wl.Frame("o", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1,
+                      grid_log2=(it["bins"].bit_length() - 1) if it["bins"] else 4)
+        o = oracle.run(fr)
+        assert np.array_equal(it["shadow"].download(np.float32).view(np.uint32), o.shadow.view(np.uint32)), k
+        assert np.array_equal(it["masks"].download(np.uint32), o.masks), k
+    b.close()
+    ctx.close()

@@ -103,6 +103,86 @@ def gpu_plain(pkg, frame, steps, warmup, quick=False):
                 stats=st.as_dict(), shadow=shadow, masks=masks)
 
 
+def gpu_batched(pkg, reps=3):
+    """batch-64: 64 instances of uneven resolution / light grid / light pose
+    over the heightfield-1k scene.  Plain = each instance through the
+    per-frame path (fd_launch, blocking like FDLaunch, and back-to-back
+    fd_launch_async); batched = one graph-replayed launch per stage."""
+    fd, wl = pkg.fd4d, pkg.workloads
+    frames = wl.batch_instances(64)
+    scene = frames[0].scene
+    strm = fd.Stream()
+    ctx = fd.Context()
+    ctx.set_backface_cull(1)
+    ctx.set_scenes_esp(frames[0].eps)
+    verts = fd.DeviceBuffer.from_array(scene.verts)
+    idx = fd.DeviceBuffer.from_array(scene.idx)
+    ctx.add_model(verts, scene.nverts, 16, idx, scene.ntris)
+    rxbuf = {}
+    items = []
+    bits = 0
+    for f in frames:
+        key = (f.rx.w, f.rx.h)
+        if key not in rxbuf:
+            rxbuf[key] = (fd.DeviceBuffer.from_array(f.rx.pos), fd.DeviceBuffer.from_array(f.rx.nrm))
+        R = f.light.res_w
+        items.append(dict(w=f.rx.w, h=f.rx.h, light=f.light, pos=rxbuf[key][0], nrm=rxbuf[key][1], bins=0,
+                          shadow=fd.DeviceBuffer(f.rx.w * f.rx.h * 4),
+                          masks=fd.DeviceBuffer(f.rx.w * f.rx.h * fd.mask_words(R) * 4)))
+        bits += f.rx.w * f.rx.h * R * R
+    # plain per-frame path through one context (sized by a warm-up pass)
+    def plain(sync_each):
+        for it in items:
+            ctx.set_receivers(it["pos"], it["nrm"], it["w"], it["h"])
+            ctx.set_outputs(it["shadow"], it["masks"])
+            ctx.set_area_light(it["light"])
+            if sync_each:
+                rc, _ = ctx.launch(strm.handle)
+                assert rc == 0
+            else:
+                ctx.launch_async(strm.handle)
+        strm.sync()
+    plain(True)
+    ref = [(it["shadow"].download(np.float32), it["masks"].download(np.uint32)) for it in items]
+    t = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        plain(True)
+        t.append(time.perf_counter() - a)
+    plain_sync = min(t)
+    t = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        plain(False)
+        t.append(time.perf_counter() - a)
+    plain_async = min(t)
+    # batched, graph-captured
+    b = fd.Batch(ctx, items)
+    b.capture(strm.handle)
+    b.launch(strm.handle)
+    strm.sync()
+    exact = all(np.array_equal(it["shadow"].download(np.float32).view(np.uint32), r[0].view(np.uint32)) and
+                np.array_equal(it["masks"].download(np.uint32), r[1]) for it, r in zip(items, ref))
+    t = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        b.launch(strm.handle)
+        strm.sync()
+        t.append(time.perf_counter() - a)
+    batched = min(t)
+    b.close()
+    ctx.close()
+    gb = bits / 1e9
+    return dict(workload="batch-64 (heightfield-1k scene, 200k triangles; receivers from {256^2, 512^2, 1024^2, "
+                "1920x1080}, light grids from {8^2, 16^2, 32^2}, poses U[-2,2]^2 x [4,8])",
+                instances=len(items), gbit_per_batch=round(gb, 4),
+                plain_sync_ms=round(plain_sync * 1e3, 2), plain_async_ms=round(plain_async * 1e3, 2),
+                batched_graph_ms=round(batched * 1e3, 2),
+                plain_sync_gbps=round(gb / plain_sync, 3), plain_async_gbps=round(gb / plain_async, 3),
+                batched_gbps=round(gb / batched, 3), speedup_vs_plain_sync=round(plain_sync / batched, 3),
+                speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact))
+
+
 def cpu_baseline(frame, gpu, seconds):
     """The oracle (CPU restatement of the reference) on a receiver sample,
     all host cores; full-frame time = fixed stages + occlusion scaled."""
@@ -148,6 +228,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="timed frames only (for profiler runs)")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
@@ -189,6 +270,9 @@ def main():
                frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
                                 valid_triangles=g["stats"]["valid_triangles"]),
                roofline=roofline)
+    if not args.no_batch and not args.quick:
+        log("[bench] batched form (batch-64)")
+        out["batched"] = gpu_batched(pkg)
     if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)

@@ -13,8 +13,9 @@
 #include "../../include/fd4d.h"
 #include "fd_kernels.h"
 #include "fd_types.h"
+#include "fd_host.h"
 
-namespace {
+namespace fdh {
 
 // ---------------------------------------------------------------------------
 // Light set-up on the host (float arithmetic in the reference's order;
@@ -78,29 +79,9 @@ void light_setup(const fd_area_light& d, FdLight& L) {
     L.wpr = (W <= 32) ? 1 : 2;
 }
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
-        size_t want = bytes ? bytes : 256;
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
-};
+}  // namespace fdh
 
-struct Model {
-    const float* verts;
-    int vstride;
-    int nverts;
-    const int* idx;
-    int ntris;
-};
-
-}  // namespace
+using namespace fdh;
 
 struct fd_ctx {
     int device = 0;
@@ -130,9 +111,7 @@ struct fd_ctx {
     hipEvent_t stageEv[FD_NSTAGES + 1] = {};
 };
 
-namespace {
-
-inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+namespace fdh {
 
 int hip_status(hipError_t e) {
     if (e == hipSuccess) return FD_OK;
@@ -209,6 +188,10 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
 }
 
 // Size the workspace for the current view / scene / grid and fill FdArgs.
+}  // namespace fdh
+
+namespace {
+
 int prepare(fd_ctx* c, FdArgs& a) {
     if (c->models.empty() || !c->pos || !c->nrm || !c->shadow || c->W <= 0 || c->H <= 0 || !c->lightSet)
         return FD_ERR_STATE;
@@ -223,6 +206,10 @@ int prepare(fd_ctx* c, FdArgs& a) {
     layout(a, (char*)c->arena.p, c->countOps);
     return FD_OK;
 }
+
+}  // namespace
+
+namespace fdh {
 
 void fill_stats(const FdState& s, fd_stats* o) {
     std::memset(o, 0, sizeof(*o));
@@ -239,6 +226,10 @@ void fill_stats(const FdState& s, fd_stats* o) {
     o->light_plane_end[1] = s.E[1];
     o->far_plane = s.f;
 }
+
+}  // namespace fdh
+
+namespace {
 
 struct M16 { float m[16]; };
 
@@ -527,6 +518,77 @@ const char* fd_status_string(int code) {
 // context's scene, one launch per stage for all of them, optionally captured
 // into a HIP graph.
 // ---------------------------------------------------------------------------
+namespace fdh {
+
+size_t plan_bytes(int n) {
+    // instance table + two occlusion tables + maps (n + 1 ints each), 256-aligned
+    return ((sizeof(FdArgs) * (size_t)(2 * n) + 255) & ~(size_t)255) + (size_t)FD_NMAPS * (n + 1) * sizeof(int) + 256;
+}
+
+size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, std::vector<char>& image,
+                  FdBatchPlan& P) {
+    const int n = (int)args.size();
+    P = FdBatchPlan{};
+    P.n = n;
+    P.maxNb = 0;
+    P.anyTris = anyTris;
+    std::vector<int> occIdx[2];
+    for (int i = 0; i < n; ++i) {
+        P.maxNb = std::max(P.maxNb, 1 << (2 * args[i].gridLog2));
+        occIdx[args[i].L.wpr - 1].push_back(i);
+    }
+    for (int cl = 0; cl < 2; ++cl) {  // largest work first: shorter tail
+        auto cost = [&](int i) { return (double)args[i].WH * args[i].L.res * args[i].L.res; };
+        std::stable_sort(occIdx[cl].begin(), occIdx[cl].end(), [&](int x, int y) { return cost(x) > cost(y); });
+        P.nocc[cl] = (int)occIdx[cl].size();
+        P.maxRes[cl] = 1;
+        for (int i : occIdx[cl]) P.maxRes[cl] = std::max(P.maxRes[cl], args[i].L.res);
+    }
+    const size_t argBytes = (sizeof(FdArgs) * (size_t)(2 * n) + 255) & ~(size_t)255;
+    image.assign(argBytes + (size_t)FD_NMAPS * (n + 1) * sizeof(int), 0);
+    FdArgs* hArgs = (FdArgs*)image.data();
+    for (int i = 0; i < n; ++i) hArgs[i] = args[i];
+    int k = n;
+    for (int cl = 0; cl < 2; ++cl)
+        for (int i : occIdx[cl]) hArgs[k++] = args[i];
+    int* hMaps = (int*)(image.data() + argBytes);
+    const FdArgs* dArgs = (const FdArgs*)devBase;
+    const int* dMaps = (const int*)(devBase + argBytes);
+    P.inst = dArgs;
+    P.occ[0] = dArgs + n;
+    P.occ[1] = dArgs + n + P.nocc[0];
+    for (int m = 0; m < FD_NMAPS; ++m) {
+        const bool occ = (m == FD_MAP_OCC32 || m == FD_MAP_OCC64);
+        const std::vector<int>* order = occ ? &occIdx[m == FD_MAP_OCC64 ? 1 : 0] : nullptr;
+        const int cnt = occ ? (int)order->size() : n;
+        int* mp = hMaps + (size_t)m * (n + 1);
+        P.map[m] = dMaps + (size_t)m * (n + 1);
+        int acc = 0;
+        mp[0] = 0;
+        for (int q = 0; q < cnt; ++q) {
+            const FdArgs& a = args[occ ? (*order)[q] : q];
+            const int nb = 1 << (2 * a.gridLog2);
+            int blocks = 1;
+            switch (m) {
+                case FD_MAP_ONE: blocks = 1; break;
+                case FD_MAP_BINS: blocks = cdiv(nb, 256); break;
+                case FD_MAP_RED: blocks = a.nbR + a.nbV; break;
+                case FD_MAP_RX: blocks = a.nblkR; break;
+                case FD_MAP_SCANR: blocks = fdk::scan_chunks(a.nR); break;
+                case FD_MAP_TRI: blocks = a.nblkT; break;
+                case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
+                default: blocks = cdiv(a.WH, 64) + nb + 1; break;
+            }
+            acc += blocks;
+            mp[q + 1] = acc;
+        }
+        P.total[m] = acc;
+    }
+    return image.size();
+}
+
+}  // namespace fdh
+
 struct fd_batch {
     fd_ctx* ctx = nullptr;
     int n = 0;
@@ -572,69 +634,11 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
     hipError_t e = b->arena.ensure(off[n]);
     if (e != hipSuccess) { delete b; return hip_status(e); }
     for (int i = 0; i < n; ++i) layout(b->args[i], (char*)b->arena.p + off[i], false);
-    // block maps
-    FdBatchPlan& P = b->plan;
-    P.n = n;
-    P.maxNb = 0;
-    P.anyTris = md.ntris > 0;
-    std::vector<int> occIdx[2];
-    for (int i = 0; i < n; ++i) {
-        P.maxNb = std::max(P.maxNb, 1 << (2 * b->args[i].gridLog2));
-        occIdx[b->args[i].L.wpr - 1].push_back(i);
-    }
-    for (int cl = 0; cl < 2; ++cl) {  // largest work first: shorter tail
-        auto cost = [&](int i) { return (double)b->args[i].WH * b->args[i].L.res * b->args[i].L.res; };
-        std::stable_sort(occIdx[cl].begin(), occIdx[cl].end(), [&](int x, int y) { return cost(x) > cost(y); });
-        P.nocc[cl] = (int)occIdx[cl].size();
-        P.maxRes[cl] = 1;
-        for (int i : occIdx[cl]) P.maxRes[cl] = std::max(P.maxRes[cl], b->args[i].L.res);
-    }
-    std::vector<int> maps;
-    int mapOff[FD_NMAPS];
-    for (int m = 0; m < FD_NMAPS; ++m) {
-        mapOff[m] = (int)maps.size();
-        const bool occ = (m == FD_MAP_OCC32 || m == FD_MAP_OCC64);
-        const std::vector<int>* order = occ ? &occIdx[m == FD_MAP_OCC64 ? 1 : 0] : nullptr;
-        const int cnt = occ ? (int)order->size() : n;
-        int acc = 0;
-        maps.push_back(0);
-        for (int k = 0; k < cnt; ++k) {
-            const FdArgs& a = b->args[occ ? (*order)[k] : k];
-            const int nb = 1 << (2 * a.gridLog2);
-            int blocks = 1;
-            switch (m) {
-                case FD_MAP_ONE: blocks = 1; break;
-                case FD_MAP_BINS: blocks = cdiv(nb, 256); break;
-                case FD_MAP_RED: blocks = a.nbR + a.nbV; break;
-                case FD_MAP_RX: blocks = a.nblkR; break;
-                case FD_MAP_SCANR: blocks = fdk::scan_chunks(a.nR); break;
-                case FD_MAP_TRI: blocks = a.nblkT; break;
-                case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
-                default: blocks = cdiv(a.WH, 64) + nb + 1; break;
-            }
-            acc += blocks;
-            maps.push_back(acc);
-        }
-        P.total[m] = acc;
-    }
-    std::vector<FdArgs> occTab[2];
-    for (int cl = 0; cl < 2; ++cl)
-        for (int i : occIdx[cl]) occTab[cl].push_back(b->args[i]);
-    const size_t argBytes = sizeof(FdArgs) * (size_t)(n + P.nocc[0] + P.nocc[1]);
-    e = b->tables.ensure(argBytes + maps.size() * sizeof(int));
+    e = b->tables.ensure(fdh::plan_bytes(n));
     if (e == hipSuccess) {
-        FdArgs* dArgs = (FdArgs*)b->tables.p;
-        e = hipMemcpy(dArgs, b->args.data(), n * sizeof(FdArgs), hipMemcpyHostToDevice);
-        if (e == hipSuccess && P.nocc[0])
-            e = hipMemcpy(dArgs + n, occTab[0].data(), P.nocc[0] * sizeof(FdArgs), hipMemcpyHostToDevice);
-        if (e == hipSuccess && P.nocc[1])
-            e = hipMemcpy(dArgs + n + P.nocc[0], occTab[1].data(), P.nocc[1] * sizeof(FdArgs), hipMemcpyHostToDevice);
-        int* dMaps = (int*)((char*)b->tables.p + argBytes);
-        if (e == hipSuccess) e = hipMemcpy(dMaps, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice);
-        P.inst = dArgs;
-        P.occ[0] = dArgs + n;
-        P.occ[1] = dArgs + n + P.nocc[0];
-        for (int m = 0; m < FD_NMAPS; ++m) P.map[m] = dMaps + mapOff[m];
+        std::vector<char> image;
+        fdh::build_plan(b->args, md.ntris > 0, (char*)b->tables.p, image, b->plan);
+        e = hipMemcpy(b->tables.p, image.data(), image.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) { delete b; return hip_status(e); }
     *out = b;

@@ -58,6 +58,25 @@ class Instance(ctypes.Structure):
                 ("max_pairs", ctypes.c_ulonglong)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("streams", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("bins_per_side", ctypes.c_int), ("backface_cull", ctypes.c_int), ("eps", ctypes.c_float),
+                ("pad", ctypes.c_int), ("max_pairs", ctypes.c_ulonglong)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int), ("pad", ctypes.c_int),
+                ("d_position4", ctypes.c_void_p), ("d_normal4", ctypes.c_void_p), ("light", AreaLight),
+                ("d_shadow", ctypes.c_void_p), ("d_masks", ctypes.c_void_p), ("user_tag", ctypes.c_ulonglong)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("user_tag", ctypes.c_ulonglong), ("status", ctypes.c_int),
+                ("exit_code", ctypes.c_int), ("batch_size", ctypes.c_int), ("pad", ctypes.c_int),
+                ("pairs", ctypes.c_ulonglong), ("t_submit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double),
+                ("t_done_ms", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -100,6 +119,13 @@ def lib():
             "fd_batch_launch": ([vp, vp], ip),
             "fd_batch_get_stats": ([vp, ip, ctypes.POINTER(Stats)], ip),
             "fd_batch_destroy": ([vp], ip),
+            "fd_server_create": ([ctypes.c_void_p, ctypes.POINTER(vp)], ip),
+            "fd_server_add_scene": ([vp, vp, sz, ip, vp, ip], ip),
+            "fd_server_submit": ([vp, ctypes.c_void_p], ctypes.c_longlong),
+            "fd_server_poll": ([vp, ctypes.c_void_p, ip], ip),
+            "fd_server_drain": ([vp, ip], ip),
+            "fd_server_now_ms": ([vp], ctypes.c_double),
+            "fd_server_destroy": ([vp], ip),
             "fd_dev_alloc": ([sz, ctypes.POINTER(vp)], ip),
             "fd_dev_free": ([vp], ip),
             "fd_memcpy_htod": ([vp, vp, sz], ip),
@@ -385,6 +411,55 @@ class Batch:
     def close(self):
         if self.h:
             lib().fd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving form: resident scenes, continuous batching over streams."""
+
+    def __init__(self, budget_bytes, streams=4, max_batch=16, bins=16, cull=1, eps=1e-3, max_pairs=0):
+        cfg = ServerConfig(budget_bytes, streams, max_batch, bins, cull, eps, 0, max_pairs)
+        h = ctypes.c_void_p()
+        _check(lib().fd_server_create(ctypes.cast(ctypes.pointer(cfg), ctypes.c_void_p), ctypes.byref(h)),
+               "fd_server_create")
+        self.h = h
+        self._keep = []
+
+    def add_scene(self, verts_dev, nverts, stride_bytes, idx_dev, ntris):
+        sid = lib().fd_server_add_scene(self.h, _ptr(verts_dev), stride_bytes, nverts, _ptr(idx_dev), ntris)
+        if sid < 0:
+            raise FdError(f"fd_server_add_scene -> {sid}")
+        self._keep += [verts_dev, idx_dev]
+        return sid
+
+    def submit(self, scene, pos, nrm, w, h, light, shadow, masks=None, tag=0):
+        r = Request(scene, w, h, 0, pos.ptr, nrm.ptr, AreaLight.from_desc(light), shadow.ptr,
+                    masks.ptr if masks is not None else 0, tag)
+        rid = lib().fd_server_submit(self.h, ctypes.cast(ctypes.pointer(r), ctypes.c_void_p))
+        if rid < 0:
+            raise FdError(f"fd_server_submit -> {rid}")
+        return rid
+
+    def poll(self, max_n=256):
+        arr = (Completion * max_n)()
+        n = lib().fd_server_poll(self.h, ctypes.cast(arr, ctypes.c_void_p), max_n)
+        return [arr[i] for i in range(n)]
+
+    def drain(self, timeout_ms=60000):
+        _check(lib().fd_server_drain(self.h, timeout_ms), "fd_server_drain")
+
+    def now_ms(self):
+        return lib().fd_server_now_ms(self.h)
+
+    def close(self):
+        if self.h:
+            lib().fd_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

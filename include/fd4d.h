@@ -175,6 +175,58 @@ int fd_batch_launch(fd_batch* batch, void* hip_stream);
 int fd_batch_get_stats(fd_batch* batch, int i, fd_stats* stats);
 int fd_batch_destroy(fd_batch* batch);
 
+/* ---- Serving form (no counterpart).  Resident scenes; requests of uneven
+ * size arrive at any time; a worker thread admits them in arrival order as
+ * batch slots (streams) and HBM (the workspace pool) free up, runs each
+ * admitted group as one batched launch, and overlaps batches across streams.
+ * Results are bit-identical to fd_launch of the request alone.  Device
+ * buffers named in a request must stay valid until its completion is
+ * polled. */
+typedef struct fd_server fd_server;
+typedef struct fd_server_config {
+    size_t hbm_budget_bytes;  /* workspace pool (>= 64 MiB) */
+    int streams;              /* batches in flight (default 4) */
+    int max_batch;            /* requests per batch (default 16) */
+    int bins_per_side;        /* default 16 */
+    int backface_cull;
+    float eps;
+    int pad;
+    unsigned long long max_pairs; /* per request; 0 = 16M */
+} fd_server_config;
+typedef struct fd_request {
+    int scene;                /* id from fd_server_add_scene */
+    int width, height;
+    int pad;
+    const float* d_position4;
+    const float* d_normal4;
+    fd_area_light light;
+    float* d_shadow;
+    uint32_t* d_masks;        /* or NULL */
+    unsigned long long user_tag;
+} fd_request;
+typedef struct fd_completion {
+    long long id;
+    unsigned long long user_tag;
+    int status;               /* FD_OK or FD_ERR_CAPACITY */
+    int exit_code;            /* FD_EXIT_* */
+    int batch_size;           /* requests in the batch that ran it */
+    int pad;
+    unsigned long long pairs;
+    double t_submit_ms, t_start_ms, t_done_ms;  /* fd_server_now_ms clock */
+} fd_completion;
+int fd_server_create(const fd_server_config* cfg, fd_server** out);
+/* returns the scene id (>= 0) */
+int fd_server_add_scene(fd_server* srv, const float* d_vertices, size_t vertex_size_bytes, int num_vertices,
+                        const int* d_indices, int num_triangles);
+/* returns the request id (>= 0); FD_ERR_CAPACITY if it can never fit */
+long long fd_server_submit(fd_server* srv, const fd_request* req);
+/* non-blocking; copies up to max completions, returns how many */
+int fd_server_poll(fd_server* srv, fd_completion* out, int max);
+/* waits until every submitted request completed (FD_ERR_STATE on timeout) */
+int fd_server_drain(fd_server* srv, int timeout_ms);
+double fd_server_now_ms(fd_server* srv);
+int fd_server_destroy(fd_server* srv);
+
 /* No counterpart (the reference times the whole FDLaunch with cudaEvents,
  * FD/api.cpp:405-409,574-581): per-stage device times of the next frames
  * (light plane, receiver binning, triangle pairing, tables, occlusion) and

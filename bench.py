@@ -25,6 +25,7 @@ import sys
 import time
 
 import numpy as np
+import xxhash
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -183,6 +184,125 @@ def gpu_batched(pkg, reps=3):
                 speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact))
 
 
+def _digest(bufs):
+    h = xxhash.xxh3_128()
+    for b in bufs:
+        h.update(b.download(np.uint8))
+    return h.hexdigest()
+
+
+def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
+    """serve-poisson: Poisson arrivals at `rate` req/s over 4 resident sphere
+    scenes (100k/300k/1M/2M triangles); resolution from {256^2, 512^2,
+    1024^2, 1920x1080}, light grid 16^2 or 32^2, light pose U[-2,2]^2 x [4,8].
+    Served = the continuous-batching server; plain = each request alone
+    through fd_launch, queued FIFO on the same arrival times (its latency is
+    computed from the measured per-request frame times)."""
+    fd, wl = pkg.fd4d, pkg.workloads
+    scenes = wl.serve_scenes()
+    reqs = wl.poisson_requests(nreq, rate_hz=rate)
+    dev_scenes = [(fd.DeviceBuffer.from_array(sc.verts), fd.DeviceBuffer.from_array(sc.idx)) for sc in scenes]
+    rx = {}
+    for q in reqs:
+        key = (q["w"], q["h"])
+        if key not in rx:
+            r = wl.receivers_heightfield(q["w"], q["h"], 8.0, 8.0 * q["h"] / q["w"])
+            rx[key] = (r, fd.DeviceBuffer.from_array(r.pos), fd.DeviceBuffer.from_array(r.nrm))
+    outs = []
+    bits = 0
+    for q in reqs:
+        R = q["grid"]
+        outs.append((fd.DeviceBuffer(q["w"] * q["h"] * 4), fd.DeviceBuffer(q["w"] * q["h"] * fd.mask_words(R) * 4)))
+        bits += q["w"] * q["h"] * R * R
+    lights = [wl.LightDesc(1.0, 1.0, q["pose"], res_w=q["grid"], res_h=q["grid"]) for q in reqs]
+    # plain per-request path: one context per scene, frame times measured alone
+    strm = fd.Stream()
+    ctxs = []
+    for (v, i), sc in zip(dev_scenes, scenes):
+        c = fd.Context()
+        c.set_backface_cull(1)
+        c.set_scenes_esp(1e-3)
+        c.add_model(v, sc.nverts, 16, i, sc.ntris)
+        ctxs.append(c)
+    svc = []
+    ref = []
+    for k, q in enumerate(reqs):
+        c = ctxs[q["scene"]]
+        r, pos, nrm = rx[(q["w"], q["h"])]
+        c.set_receivers(pos, nrm, q["w"], q["h"])
+        c.set_outputs(*outs[k])
+        c.set_area_light(lights[k])
+        if k == 0:
+            c.launch(strm.handle)
+        a = time.perf_counter()
+        rc, st = c.launch(strm.handle)
+        svc.append((time.perf_counter() - a) * 1e3)
+        assert rc == 0
+        ref.append(_digest(outs[k]))
+    for c in ctxs:
+        c.close()
+    arr = [q["t"] * 1e3 for q in reqs]
+    t_free, lat_plain = 0.0, []
+    for a, sv in zip(arr, svc):
+        start = max(a, t_free)
+        t_free = start + sv
+        lat_plain.append(t_free - a)
+    plain_span = t_free - arr[0]
+    # served
+    srv = fd.Server(budget_bytes=budget_gb << 30, streams=4, max_batch=16, eps=1e-3)
+    for (v, i), sc in zip(dev_scenes, scenes):
+        srv.add_scene(v, sc.nverts, 16, i, sc.ntris)
+    for o in outs:
+        o[0].upload(np.zeros(o[0].nbytes, np.uint8))
+        o[1].upload(np.zeros(o[1].nbytes, np.uint8))
+    t0 = srv.now_ms()
+    for k, q in enumerate(reqs):
+        target = t0 + arr[k]
+        while srv.now_ms() < target:
+            dt = (target - srv.now_ms()) / 1e3
+            if dt > 0.0005:
+                time.sleep(dt - 0.0003)
+        r, pos, nrm = rx[(q["w"], q["h"])]
+        srv.submit(q["scene"], pos, nrm, q["w"], q["h"], lights[k], outs[k][0], outs[k][1], tag=k)
+    srv.drain(600000)
+    done = sorted(srv.poll(len(reqs) + 8), key=lambda c: c.user_tag)
+    srv.close()
+    lat = [c.t_done_ms - c.t_submit_ms for c in done]
+    span = max(c.t_done_ms for c in done) - min(c.t_submit_ms for c in done)
+    exact = all(_digest(outs[k]) == ref[k] and done[k].status == 0 for k in range(len(reqs)))
+    gb = bits / 1e9
+    return dict(workload=f"serve-poisson ({nreq} requests at {rate:.0f} req/s; scenes 100k/300k/1M/2M triangles; "
+                "receivers from {256^2, 512^2, 1024^2, 1920x1080}; light grids 16^2 / 32^2)",
+                requests=len(reqs), gbit=round(gb, 3), offered_span_ms=round(arr[-1] - arr[0], 1),
+                served_span_ms=round(span, 1), plain_span_ms=round(plain_span, 1),
+                served_gbps=round(gb / (span * 1e-3), 3), plain_gbps=round(gb / (plain_span * 1e-3), 3),
+                speedup=round(plain_span / span, 3),
+                served_p50_ms=round(pctl(lat, 50), 2), served_p99_ms=round(pctl(lat, 99), 2),
+                plain_p50_ms=round(pctl(lat_plain, 50), 2), plain_p99_ms=round(pctl(lat_plain, 99), 2),
+                plain_frame_ms_mean=round(float(np.mean(svc)), 2),
+                mean_batch=round(float(np.mean([c.batch_size for c in done])), 2),
+                masks_bit_exact_vs_plain=bool(exact))
+
+
+def gpu_target_config(pkg, steps=10):
+    """The latency target configuration: 1920x1080 receivers (boxes G-buffer),
+    32x32 light samples, 1M triangles; plain path, one frame at a time."""
+    fd, wl = pkg.fd4d, pkg.workloads
+    frame = wl.boxes_1080p(light_res=32)
+    strm = fd.Stream()
+    run = fd.FrameRunner(frame, stream=strm)
+    run.run()
+    lat = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        rc, st = run.run()
+        lat.append((time.perf_counter() - a) * 1e3)
+    run.close()
+    return dict(workload="boxes-1080p with a 32x32 light (1920x1080 receivers, 1M triangles)",
+                p50_ms=round(pctl(lat, 50), 2), p99_ms=round(pctl(lat, 99), 2), exit=st.exit_code,
+                gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
+
+
 def cpu_baseline(frame, gpu, seconds):
     """The oracle (CPU restatement of the reference) on a receiver sample,
     all host cores; full-frame time = fixed stages + occlusion scaled."""
@@ -229,6 +349,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="timed frames only (for profiler runs)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
@@ -273,6 +394,11 @@ def main():
     if not args.no_batch and not args.quick:
         log("[bench] batched form (batch-64)")
         out["batched"] = gpu_batched(pkg)
+    if not args.no_serve and not args.quick:
+        log("[bench] serving form (serve-poisson)")
+        out["served"] = gpu_served(pkg)
+        log("[bench] latency target configuration")
+        out["target_1080p_32x32_1M"] = gpu_target_config(pkg)
     if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)

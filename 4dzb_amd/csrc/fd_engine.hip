@@ -693,11 +693,11 @@ extern "C" int fd_batch_destroy(fd_batch* b) {
 // ---------------------------------------------------------------------------
 namespace fdk {
 hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st);
-hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out);
+hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, int mode, unsigned long long* out);
 }
 
-extern "C" int fd_selftest_rowmask(const float* hD, const float* hB, int n, int R, unsigned long long* hout) {
-    if (!hD || !hB || !hout || n <= 0 || R < 1 || R > 64) return FD_ERR_ARG;
+extern "C" int fd_selftest_rowmask(const float* hD, const float* hB, int n, int R, int mode, unsigned long long* hout) {
+    if (!hD || !hB || !hout || n <= 0 || R < 1 || R > 64 || mode < 0 || mode > 1) return FD_ERR_ARG;
     float *dD = nullptr, *dB = nullptr;
     unsigned long long* dO = nullptr;
     hipError_t e = hipMalloc(&dD, n * sizeof(float));
@@ -705,7 +705,7 @@ extern "C" int fd_selftest_rowmask(const float* hD, const float* hB, int n, int 
     if (e == hipSuccess) e = hipMalloc(&dO, n * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemcpy(dD, hD, n * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dB, hB, n * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = fdk::selftest_rowmask(dD, dB, n, R, dO);
+    if (e == hipSuccess) e = fdk::selftest_rowmask(dD, dB, n, R, mode, dO);
     if (e == hipSuccess) e = hipMemcpy(hout, dO, n * sizeof(unsigned long long), hipMemcpyDeviceToHost);
     if (dD) (void)hipFree(dD);
     if (dB) (void)hipFree(dB);

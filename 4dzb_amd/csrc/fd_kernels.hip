@@ -752,17 +752,21 @@ __device__ __forceinline__ int k_exact(float D, float B, int R) {
     return (int)c;
 }
 
+// One edge plane A*row + B*col + C over the R x R sample grid, negated when
+// the receiver sees the triangle's back (see the set-up in occlude_body).
 struct Edge {
-    float A, B, C;    // plane value A*row + B*col + C
-    float rowN;       // A * (R-1)   (f2 of the row-range mask)
-    float colN;       // B * (R-1)   (f2 of a row mask)
-    float rA, rB;     // reciprocals for the fast quotient
-    bool okA, okB;    // |A| / |B| inside [2^-100, 2^100]
+    float A, B, C;
+    float nRowN;      // -(A * (R-1)):  f2 of the row-range mask is DN >= nRowN
+    float nColN;      // -(B * (R-1)):  f2 of a row mask is D >= nColN
+    float rA, rB;     // rcp(A), rcp(B); NaN when |A| / |B| is outside [2^-100, 2^100]
 };
 
 __device__ __forceinline__ bool rcp_ok(float x) {
     const float ax = fabsf(x);
     return ax >= 7.8886091e-31f && ax <= 1.2676506e+30f;
+}
+__device__ __forceinline__ float rcp_or_nan(float x) {
+    return rcp_ok(x) ? __builtin_amdgcn_rcpf(x) : __builtin_nanf("");
 }
 
 // kk for one (D, divisor) in the division branch; sets slow when the exact
@@ -770,10 +774,11 @@ __device__ __forceinline__ bool rcp_ok(float x) {
 // quotient q is within 2e-5 of the exact one for q <= R + 1, and the exact
 // quotient is >= 0 in this branch, so q > -2e-5: anything within 2^-12 of an
 // integer (0 included) goes to the exact path.  q > R + 1 means all columns.
-__device__ __forceinline__ int k_fast(float D, float r, bool ok, float Rf, bool& slow) {
+// A NaN reciprocal (divisor out of range) makes q NaN: always exact.
+__device__ __forceinline__ int k_fast(float D, float r, float Rf, bool& slow) {
     const float q = (-D) * r;
     const float dist = fabsf(q - rintf(q));
-    const bool fine = ok && (dist > 2.44140625e-4f || q > Rf + 1.0f);
+    const bool fine = dist > 2.44140625e-4f || q > Rf + 1.0f;
     slow = !fine;
     return (int)__builtin_amdgcn_fmed3f(ceilf(q), 0.0f, Rf);   // NaN -> 0, then in [0, R]
 }
@@ -788,6 +793,7 @@ __device__ __forceinline__ T low_mask(int kk) {
 // (!f1 && !f2), a prefix [0, kk) (f1 && !f2) or a suffix [kk, R) (!f1 && f2);
 // with v = (f1 != f2) ? kk : R the AND is the interval
 // [max over !f1 edges of v, min over f1 edges of v).
+// General form: valid for every input, including infinite limits B*(R-1).
 template <typename T, bool ROWS>
 __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3], float Rf, int R) {
     bool f1[3], f2[3], slow[3];
@@ -796,8 +802,8 @@ __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3]
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         f1[k] = D[k] >= 0.0f;
-        f2[k] = ((ROWS ? e[k].rowN : e[k].colN) + D[k]) >= 0.0f;
-        kk[k] = k_fast(D[k], ROWS ? e[k].rA : e[k].rB, ROWS ? e[k].okA : e[k].okB, Rf, slow[k]);
+        f2[k] = (D[k] - (ROWS ? e[k].nRowN : e[k].nColN)) >= 0.0f;   // fl(B*n + D) >= 0
+        kk[k] = k_fast(D[k], ROWS ? e[k].rA : e[k].rB, Rf, slow[k]);
         slow[k] = slow[k] && (f1[k] != f2[k]);
         anySlow |= slow[k];
     }
@@ -812,6 +818,135 @@ __device__ __forceinline__ T three_edges(const Edge (&e)[3], const float (&D)[3]
     const int hi = min(min(f1[0] ? v[0] : R, f1[1] ? v[1] : R), f1[2] ? v[2] : R);
     const int lo = max(max(f1[0] ? 0 : v[0], f1[1] ? 0 : v[1]), f1[2] ? 0 : v[2]);
     return low_mask<T>(hi) & ~low_mask<T>(lo);
+}
+
+// Bits [lo, lo + w) of a width-bit word, w + lo <= width.
+template <typename T>
+__device__ __forceinline__ T interval_bits(unsigned w, unsigned lo) {
+    if (Bits<T>::width == 32) {
+        unsigned m;
+        asm("v_bfm_b32 %0, %1, %2" : "=v"(m) : "v"(w), "v"(lo));   // ((1 << w[4:0]) - 1) << lo[4:0]
+        return (w >= 32u) ? ~(T)0 : (T)m;
+    }
+    return (w >= 64u) ? ~(T)0 : (T)((((uint64_t)1 << w) - 1ull) << (lo & 63u));
+}
+
+// Integer max3 / min3 as single instructions (kept opaque so that the
+// selects feeding them are not folded into longer select chains).
+__device__ __forceinline__ int max3i(int a, int b, int c) {
+    int r;
+    asm("v_max3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ int min3i(int a, int b, int c) {
+    int r;
+    asm("v_min3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Fast form of three_edges for limits nN = -(divisor * (R-1)) that are finite
+// in every lane of the wave (the caller checks |A|, |B| <= 2^100).
+//  * f2: for finite a and b, fl(a + b) >= 0 exactly when a + b >= 0 (a sum
+//    of two floats is 0 or at least one denormal step away from it), so
+//    fl(B*n + D) >= 0  <=>  D >= -fl(B*n)  -- also for infinite D.
+//  * kk: only read when f1 != f2.  Then the exact quotient q lies in
+//    [0, n + ulp] (f1 && !f2: D >= 0 > B; !f1 && f2: D < 0 < B, both with
+//    |D| <= |B*n|).  x = fma(D, rcp(B), R) is within |q| 2^-22 + 2^-19 <
+//    2^-12 of R - q; when x is more than 2^-12 away from every integer,
+//    floor(x) = R - ceil(q) = R - kk exactly, in [1, R-1] (no clamp).
+//    The lanes left (x near an integer, NaN reciprocal) divide exactly.
+//  * with c = R - kk the AND is [R - min over suffix edges of c,
+//    R - max over prefix edges of c), "none" edges giving the empty set.
+template <typename T>
+__device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (&nN)[3], const float (&r)[3],
+                                              const float (&div)[3], float Rf) {
+    bool f1[3], f2[3], slow[3];
+    int c[3];
+    bool anySlow = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f1[k] = D[k] >= 0.0f;
+        f2[k] = D[k] >= nN[k];
+        const float x = __builtin_fmaf(D[k], r[k], Rf);
+        asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(c[k]) : "v"(x));   // floor(x), saturating, NaN -> 0
+        slow[k] = !(fabsf(x - rintf(x)) > 2.44140625e-4f) && (f1[k] != f2[k]);
+        anySlow |= slow[k];
+    }
+    if (anySlow) {
+        const int R = (int)Rf;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (slow[k]) c[k] = R - k_exact(D[k], div[k], R);
+    }
+    const int R = (int)Rf;
+    const int hiC = max3i((f1[0] && !f2[0]) ? c[0] : 0, (f1[1] && !f2[1]) ? c[1] : 0, (f1[2] && !f2[2]) ? c[2] : 0);
+    const int loC = min3i((!f1[0] && f2[0]) ? c[0] : R, (!f1[1] && f2[1]) ? c[1] : R, (!f1[2] && f2[2]) ? c[2] : R);
+    const bool none = (!f1[0] && !f2[0]) || (!f1[1] && !f2[1]) || (!f1[2] && !f2[2]);
+    const unsigned w = __builtin_elementwise_sub_sat((unsigned)loC, (unsigned)hiC);
+    const T m = interval_bits<T>(w, (unsigned)(R - loC));
+    return none ? (T)0 : m;
+}
+
+// Rows whose three-edge mask can be non-empty.  The reference visits every
+// row of the row-range mask (each edge on its own); most of those rows end
+// with an empty AND because a prefix edge (B < 0) and a suffix edge (B > 0)
+// exclude each other.  Skipping a row whose AND is empty changes no output,
+// so the window below only has to contain every row with a non-empty AND.
+//
+// Bound.  For a row r and column c in [0, R-1], a set bit of one edge's mask
+// implies A r + B c + C >= -tau with tau = 4u (|A| R + |B| R + |C|)
+// (u = 2^-24: the rounding of A*r, of + C, of the quotient and of B*(R-1);
+// the set bit means c < q (prefix), c >= q (suffix) or both end columns
+// pass).  The code uses tau' = 2^-20 (...) + 2^-126 and C' = fl(C + tau').
+// A prefix edge m and a suffix edge p meet in some real column only if
+//   |B_m| (A_p r + C'_p) + B_p (A_m r + C'_m) >= 0,
+// which for a pair (i, j) of opposite-signed B is  s G(r) >= 0  with
+// G(r) = (B_i A_j - B_j A_i) r + (B_i C'_j - B_j C'_i) = al r + be, s = sign(B_i).
+// al and be are computed in float; E = 2^-20 (Sa R + Sb) + 2^-126 bounds
+// their rounding (Sa, Sb: sums of the absolute products), so every row with a
+// non-empty mask has  s (al r + be) + E >= 0, i.e.
+//   r >= t0 - E/|al|  (s al > 0)   or   r <= t0 + E/|al|  (s al < 0),  t0 = -be/al.
+// t0 and E/|al| come from v_rcp_f32 (2^-22 relative); a margin of 2^-10
+// covers that while |t0| <= 2^11, and a pair is used only when
+// E/|al| <= 2^10, so that beyond |t0| = 2^11 the window is far outside
+// [0, R-1] either way.  Pairs with |al| < 2^-40, and lanes with any |A|, |B|,
+// |C| above 2^40 (where the products could overflow), are not used; NaN
+// thresholds drop out of fmaxf / fminf.
+template <typename T>
+__device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf) {
+    const float u20 = 9.5367431640625e-07f;   // 2^-20
+    const float tiny = 1.17549435e-38f;       // 2^-126
+    const float lim = 1.09951163e+12f;        // 2^40
+    float Cp[3];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a = fabsf(e[k].A), b = fabsf(e[k].B), c = fabsf(e[k].C);
+        ok = ok && a <= lim && b <= lim && c <= lim;
+        Cp[k] = e[k].C + __builtin_fmaf(__builtin_fmaf(a + b, Rf, c), u20, tiny);
+    }
+    float lo = 0.0f, hi = Rf - 1.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int i = k, j = (k + 1) % 3;
+        const float pA = e[i].B * e[j].A, qA = e[j].B * e[i].A;
+        const float pB = e[i].B * Cp[j], qB = e[j].B * Cp[i];
+        const float al = pA - qA, be = pB - qB;
+        const float E = __builtin_fmaf(__builtin_fmaf(fabsf(pA) + fabsf(qA), Rf, fabsf(pB) + fabsf(qB)), u20, tiny);
+        const float ra = __builtin_amdgcn_rcpf(al);
+        const float t0 = -be * ra;
+        const float d = E * fabsf(ra);
+        const bool opp = (e[i].B > 0.0f && e[j].B < 0.0f) || (e[i].B < 0.0f && e[j].B > 0.0f);
+        const bool use = ok && opp && fabsf(al) >= 9.09494702e-13f && d <= 1024.0f;
+        const bool lower = (e[i].B > 0.0f) == (al > 0.0f);
+        const float m = d + 9.765625e-4f;
+        lo = fmaxf(lo, (use && lower) ? t0 - m : 0.0f);
+        hi = fminf(hi, (use && !lower) ? t0 + m : Rf - 1.0f);
+    }
+    const float lc = ceilf(fminf(lo, Rf));
+    const float hc = floorf(fmaxf(hi, -1.0f));
+    const unsigned w = (unsigned)fmaxf(hc + 1.0f - lc, 0.0f);
+    return interval_bits<T>(w, (unsigned)lc);
 }
 
 // test3DPointInTriangle, Step5_2_shadowCal.inl:213-222
@@ -960,6 +1095,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
         Edge e[3];
         const float* vb[3] = {t.v1, t.v2, t.v0};
         const float* ev[3] = {t.e01, t.e12, t.e20};
+        bool big = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float ax = vb[k][0] - q.q[0], ay = vb[k][1] - q.q[1], az = vb[k][2] - q.q[2];
@@ -970,18 +1106,35 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
             float C = dot3(nx, ny, nz, Lx, Ly, Lz);
             if (dir < 0) { A = -A; B = -B; C = -C; }
             e[k].A = A; e[k].B = B; e[k].C = C;
-            e[k].rowN = A * nsub;
-            e[k].colN = B * nsub;
-            e[k].rA = __builtin_amdgcn_rcpf(A);
-            e[k].rB = __builtin_amdgcn_rcpf(B);
-            e[k].okA = rcp_ok(A);
-            e[k].okB = rcp_ok(B);
+            big |= !(fabsf(A) <= 1.2676506e+30f && fabsf(B) <= 1.2676506e+30f);
+        }
+        // the fast mask form needs finite limits in every lane of the wave
+        const bool fast = !__any(big);
+        T win = FULL;
+        if (fast) {
+            win = row_window<T>(e, Rf);
+            if (!__any(win != (T)0)) continue;   // no lane can gain a bit from this triangle
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            e[k].nRowN = -(e[k].A * nsub);
+            e[k].nColN = -(e[k].B * nsub);
+            e[k].rA = rcp_or_nan(e[k].A);
+            e[k].rB = rcp_or_nan(e[k].B);
         }
         // rows that may hold occluded samples (:176-191)
         float DN[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) DN[k] = (e[k].B > 0 ? e[k].colN : 0.0f) + e[k].C;
-        T valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
+        for (int k = 0; k < 3; ++k) DN[k] = (e[k].B > 0 ? -e[k].nColN : 0.0f) + e[k].C;
+        T valid;
+        if (fast) {
+            const float nR[3] = {e[0].nRowN, e[1].nRowN, e[2].nRowN};
+            const float rA[3] = {e[0].rA, e[1].rA, e[2].rA};
+            const float dA[3] = {e[0].A, e[1].A, e[2].A};
+            valid = (~full) & win & three_edges_fast<T>(DN, nR, rA, dA, Rf);
+        } else {
+            valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
+        }
         if (COUNT) {
             // wave row iterations = max over lanes of this triangle's rows
             int nr = Bits<T>::popc(valid);
@@ -989,29 +1142,39 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
             if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) nWaveRows += nr;
         }
         // row by row, lowest row first (:204-222)
-        while (valid) {
-            if (COUNT) ++nRows;
-            const int r = Bits<T>::ctz(valid);
-            const float fr = (float)r;
-            float D[3];
+        auto rows = [&](auto rowMask) {
+            while (valid) {
+                if (COUNT) ++nRows;
+                const int r = Bits<T>::ctz(valid);
+                const float fr = (float)r;
+                float D[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) D[k] = e[k].A * fr + e[k].C;
-            const T M = FULL & three_edges<T, false>(e, D, Rf, R);
-            uint32_t* w0 = lds + r * 64 + lane;
-            T cur;
-            if (Bits<T>::width == 32) {
-                cur = (T)(*w0 | (uint32_t)M);
-                *w0 = (uint32_t)cur;
-            } else {
-                uint32_t* w1 = lds + (R + r) * 64 + lane;
-                const uint64_t mm = (uint64_t)M;
-                const uint32_t lo32 = *w0 | (uint32_t)mm, hi32 = *w1 | (uint32_t)(mm >> 32);
-                *w0 = lo32;
-                *w1 = hi32;
-                cur = (T)(((uint64_t)hi32 << 32) | lo32);
+                for (int k = 0; k < 3; ++k) D[k] = e[k].A * fr + e[k].C;
+                const T M = rowMask(D);
+                uint32_t* w0 = lds + r * 64 + lane;
+                T cur;
+                if (Bits<T>::width == 32) {
+                    cur = (T)(*w0 | (uint32_t)M);
+                    *w0 = (uint32_t)cur;
+                } else {
+                    uint32_t* w1 = lds + (R + r) * 64 + lane;
+                    const uint64_t mm = (uint64_t)M;
+                    const uint32_t lo32 = *w0 | (uint32_t)mm, hi32 = *w1 | (uint32_t)(mm >> 32);
+                    *w0 = lo32;
+                    *w1 = hi32;
+                    cur = (T)(((uint64_t)hi32 << 32) | lo32);
+                }
+                if (cur == FULL) full |= (T)1 << r;
+                valid &= valid - (T)1;
             }
-            if (cur == FULL) full |= (T)1 << r;
-            valid &= valid - (T)1;
+        };
+        if (fast) {
+            const float nC[3] = {e[0].nColN, e[1].nColN, e[2].nColN};
+            const float rB[3] = {e[0].rB, e[1].rB, e[2].rB};
+            const float dB[3] = {e[0].B, e[1].B, e[2].B};
+            rows([&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rf); });
+        } else {
+            rows([&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
         }
     }
     // fraction = popcount / (W*H) (:226-231)
@@ -1135,24 +1298,32 @@ __global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_apply(const FdArgs* __rest
 }
 
 // Verification of the row-mask primitive on given (D, B) pairs (n = R - 1),
-// through the same fast / exact paths as the occlusion kernel.
-__global__ void k_selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out) {
+// through the same fast / exact paths as the occlusion kernel.  mode 0: the
+// general form for every pair; mode 1: the fast form wherever its
+// precondition |B| <= 2^100 holds, the general form elsewhere.
+__global__ void k_selftest_rowmask(const float* D, const float* B, int n, int R, int mode, unsigned long long* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Edge e[3];
-    float DD[3];
+    float DD[3], nC[3], rB[3], dB[3];
     for (int k = 0; k < 3; ++k) {
         e[k].A = 0.0f; e[k].B = B[i]; e[k].C = 0.0f;
-        e[k].rowN = 0.0f;
-        e[k].colN = B[i] * (float)(unsigned)(R - 1);
-        e[k].rA = 0.0f; e[k].okA = false;
-        e[k].rB = __builtin_amdgcn_rcpf(B[i]);
-        e[k].okB = rcp_ok(B[i]);
+        e[k].nRowN = 0.0f;
+        e[k].nColN = -(B[i] * (float)(unsigned)(R - 1));
+        e[k].rA = __builtin_nanf("");
+        e[k].rB = rcp_or_nan(B[i]);
         DD[k] = D[i];
+        nC[k] = e[k].nColN; rB[k] = e[k].rB; dB[k] = B[i];
     }
+    const bool fast = mode == 1 && fabsf(B[i]) <= 1.2676506e+30f;
     const uint64_t FULL = (R >= 64) ? ~0ull : ((1ull << R) - 1ull);
-    if (R <= 32) out[i] = (uint64_t)(three_edges<uint32_t, false>(e, DD, (float)R, R)) & FULL;
-    else out[i] = three_edges<uint64_t, false>(e, DD, (float)R, R) & FULL;
+    if (R <= 32) {
+        out[i] = fast ? (uint64_t)three_edges_fast<uint32_t>(DD, nC, rB, dB, (float)R)
+                      : (uint64_t)(three_edges<uint32_t, false>(e, DD, (float)R, R)) & FULL;
+    } else {
+        out[i] = fast ? three_edges_fast<uint64_t>(DD, nC, rB, dB, (float)R)
+                      : three_edges<uint64_t, false>(e, DD, (float)R, R) & FULL;
+    }
 }
 
 // Stage read-back of the receiver rectangles in the reference's 2-layer
@@ -1222,8 +1393,8 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, unsigned long long* out) {
-    k_selftest_rowmask<<<cdiv(n, 256), 256>>>(D, B, n, R, out);
+hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, int mode, unsigned long long* out) {
+    k_selftest_rowmask<<<cdiv(n, 256), 256>>>(D, B, n, R, mode, out);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     return e;

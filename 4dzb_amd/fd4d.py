@@ -112,7 +112,7 @@ def lib():
             "fd_debug_read": ([vp, ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)], ip),
             "fd_status_string": ([ip], ctypes.c_char_p),
             "fd_set_profiling": ([vp, ip, ip], ip),
-            "fd_selftest_rowmask": ([vp, vp, ip, ip, vp], ip),
+            "fd_selftest_rowmask": ([vp, vp, ip, ip, ip, vp], ip),
             "fd_device_count": ([ctypes.POINTER(ip)], ip),
             "fd_batch_create": ([vp, ctypes.c_void_p, ip, ctypes.POINTER(vp)], ip),
             "fd_batch_capture": ([vp, vp], ip),
@@ -149,12 +149,13 @@ def mask_words(res):
     return lib().fd_mask_words(res)
 
 
-def selftest_rowmask(D, B, res):
-    """Device row-mask primitive on host arrays (verification only)."""
+def selftest_rowmask(D, B, res, mode=1):
+    """Device row-mask primitive on host arrays (verification only); mode 0
+    is the general form, mode 1 the fast form where it applies."""
     D = np.ascontiguousarray(D, np.float32)
     B = np.ascontiguousarray(B, np.float32)
     out = np.empty(len(D), np.uint64)
-    _check(lib().fd_selftest_rowmask(D.ctypes.data, B.ctypes.data, len(D), res, out.ctypes.data),
+    _check(lib().fd_selftest_rowmask(D.ctypes.data, B.ctypes.data, len(D), res, mode, out.ctypes.data),
            "fd_selftest_rowmask")
     return out
 

@@ -350,6 +350,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="timed frames only (for profiler runs)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--light-res", type=int, default=0, help="override the workload's light grid side")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
@@ -361,8 +362,9 @@ def main():
         sys.exit(2)
     wl = pkg.workloads
     log(f"[bench] generating {args.workload}")
-    frame = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
-        args.workload]()
+    make = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
+        args.workload]
+    frame = make(light_res=args.light_res) if args.light_res else make()
     log("[bench] GPU plain path")
     g = gpu_plain(pkg, frame, args.steps, args.warmup, args.quick)
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w

@@ -264,8 +264,10 @@ int fd_debug_read(fd_ctx* ctx, const char* name, void* host_dst, size_t max_byte
 
 /* Verification only: evaluates the light-row mask primitive of the
  * occlusion kernel (Step5_2_shadowCalPerSample.inl:14-33, clipped to a
- * res-sample row) on n host-given (D, B) pairs, one 64-bit mask each. */
-int fd_selftest_rowmask(const float* D, const float* B, int n, int res, unsigned long long* out);
+ * res-sample row) on n host-given (D, B) pairs, one 64-bit mask each.
+ * mode 0: the kernel's general form; mode 1: its fast form wherever that
+ * form's precondition (|B| <= 2^100) holds, the general form elsewhere. */
+int fd_selftest_rowmask(const float* D, const float* B, int n, int res, int mode, unsigned long long* out);
 
 const char* fd_status_string(int code);
 

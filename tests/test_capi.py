@@ -41,3 +41,19 @@ def test_host_only_calls(fd):
     assert lib.fd_set_view(None, 4, 4) == fd.FD_ERR_ARG
     assert lib.fd_launch(None, None, None) == fd.FD_ERR_ARG
     assert lib.fd_init(None) == fd.FD_ERR_ARG
+
+
+def test_fd_namespace_mirror_links(fd):
+    """include/fd_api.hpp (the FD:: mirror) compiles against the C ABI and its
+    driver program is built and linked to libfd4d.so (no GPU call)."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    inc = os.path.join(root, "include")
+    src = '#include "fd_api.hpp"\nint main() { FD::AreaLightDes d; (void)d; return FD::FDsetBinNum(16) == 0; }\n'
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", inc, "-x", "c++", "-"],
+                       input=src, capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    exe = os.path.join(root, "4dzb_amd", "fd_api_check")
+    assert os.path.exists(exe)
+    out = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True).stdout
+    for n in ["fd_init", "fd_launch", "fd_set_area_light", "fd_add_model"]:
+        assert n in out

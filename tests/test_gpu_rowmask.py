@@ -23,11 +23,12 @@ def _cases(rng, R):
     return np.concatenate([D, Dq, Ds.ravel()]), np.concatenate([B, Bq, Bs.ravel()])
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("R", [1, 2, 7, 16, 32, 33, 64])
-def test_device_rowmask_matches_oracle(fd, oracle, R):
+def test_device_rowmask_matches_oracle(fd, oracle, R, mode):
     rng = np.random.default_rng(R)
     D, B = _cases(rng, R)
-    got = fd.selftest_rowmask(D, B, R)
+    got = fd.selftest_rowmask(D, B, R, mode)
     want = np.array([oracle.rowmask(float(d), float(b), R - 1, R)[0] for d, b in zip(D, B)], np.uint64)
     bad = np.nonzero(got != want)[0]
     assert len(bad) == 0, [(float(D[i]), float(B[i]), int(got[i]), int(want[i])) for i in bad[:5]]

@@ -441,8 +441,7 @@ int fd_set_profiling(fd_ctx* c, int stage_events, int count_ops) {
     return FD_OK;
 }
 
-int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* ops4) {
-    // ops4 receives FD_NOPS counters
+int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* ops) {
     if (!c || !c->haveLast) return FD_ERR_STATE;
     if (stage_ms && c->profile) {
         hipError_t e = hipEventSynchronize(c->stageEv[FD_NSTAGES]);
@@ -450,11 +449,11 @@ int fd_get_profile(fd_ctx* c, float* stage_ms, int nstages, unsigned long long* 
         for (int i = 0; i < nstages && i < FD_NSTAGES; ++i)
             (void)hipEventElapsedTime(stage_ms + i, c->stageEv[i], c->stageEv[i + 1]);
     }
-    if (ops4) {
+    if (ops) {
         if (!c->last.opCount) return FD_ERR_STATE;
         hipError_t e = hipDeviceSynchronize();   // the frame may run on a non-blocking stream
         if (e == hipSuccess)
-            e = hipMemcpy(ops4, c->last.opCount, FD_NOPS * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+            e = hipMemcpy(ops, c->last.opCount, FD_NOPS * sizeof(unsigned long long), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_status(e);
     }
     return FD_OK;

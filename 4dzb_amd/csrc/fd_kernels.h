@@ -56,9 +56,8 @@ struct FdArgs {
     unsigned long long* opCount;  // optional occlusion counters (FD_NOPS), or null
 };
 
-// Occlusion counters: lane candidate tests, lane cull passes, lane row
-// visits, work items, wave triangle iterations, wave row iterations.
-#define FD_NOPS 6
+// Occlusion counters, in the order documented at fd_get_profile.
+#define FD_NOPS FD_PROFILE_COUNTERS
 
 // Stage boundaries recorded when profiling (events[0..FD_NSTAGES]).
 #define FD_NSTAGES 5   // light plane, receiver binning, triangle pairing, tables, occlusion

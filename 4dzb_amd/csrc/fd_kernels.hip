@@ -1059,7 +1059,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
     const float eps = a.eps;
     T full = 0;
     const int ts = a.triStart[bin], te = a.triEnd[bin];
-    unsigned long long nCand = 0, nPass = 0, nRows = 0, nWaveIt = 0, nWaveRows = 0;
+    unsigned long long nCand = 0, nPass = 0, nRows = 0, nWaveIt = 0, nWaveRows = 0, nRefRows = 0, nPlanes = 0;
     // read-only for this kernel: the constant address space lets the uniform
     // per-triangle loads go through the scalar unit (s_load)
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -1107,6 +1107,23 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
             if (dir < 0) { A = -A; B = -B; C = -C; }
             e[k].A = A; e[k].B = B; e[k].C = C;
             big |= !(fabsf(A) <= 1.2676506e+30f && fabsf(B) <= 1.2676506e+30f);
+        }
+        if (COUNT) {
+            // the rows the reference visits for this pair: its row-range mask
+            // minus the rows already full (general form, before any window)
+            ++nPlanes;
+            Edge g[3];
+            float DNg[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                g[k] = e[k];
+                g[k].nRowN = -(e[k].A * nsub);
+                g[k].nColN = -(e[k].B * nsub);
+                g[k].rA = rcp_or_nan(e[k].A);
+                g[k].rB = rcp_or_nan(e[k].B);
+                DNg[k] = (e[k].B > 0 ? -g[k].nColN : 0.0f) + e[k].C;
+            }
+            nRefRows += Bits<T>::popc((~full) & FULL & three_edges<T, true>(g, DNg, Rf, R));
         }
         // the fast mask form needs finite limits in every lane of the wave
         const bool fast = !__any(big);
@@ -1186,10 +1203,12 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
     if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
     if (COUNT) {
         // per-lane counts: nCand is wave-wide already
-        unsigned long long p = nPass, r = nRows;
+        unsigned long long p = nPass, r = nRows, rr = nRefRows, pl = nPlanes;
         for (int o = 32; o >= 1; o >>= 1) {
             p += __shfl_xor(p, o, 64);
             r += __shfl_xor(r, o, 64);
+            rr += __shfl_xor(rr, o, 64);
+            pl += __shfl_xor(pl, o, 64);
         }
         unsigned long long wr = nWaveRows;
         for (int o = 32; o >= 1; o >>= 1) wr += __shfl_xor(wr, o, 64);
@@ -1200,6 +1219,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
             atomicAdd(a.opCount + 3, 1ull);
             atomicAdd(a.opCount + 4, nWaveIt);
             atomicAdd(a.opCount + 5, wr);
+            atomicAdd(a.opCount + 6, rr);
+            atomicAdd(a.opCount + 7, pl);
         }
     }
     __syncthreads();
@@ -1222,8 +1243,24 @@ __global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) { tri_count_body
 __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) { tri_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
 __global__ void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
+// b % 8).  Work items of one bin share that bin's triangle list, so the
+// occlusion kernels renumber blocks so that each XCD takes runs of 16
+// consecutive work items (runs still alternate between XCDs, which keeps the
+// load balanced): neighbouring chunks of a bin then read the list from the
+// same L2.  A bijection of [0, n); the tail past the last full group of
+// 8 x 16 blocks keeps its order.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    constexpr int G = 16;
+    if (b >= n - n % (8 * G)) return b;
+    const int j = b % (8 * G);
+    return b - j + (j & 7) * G + (j >> 3);
+}
+
 template <typename T, bool COUNT>
-__global__ __launch_bounds__(64) void k_occlude(FdArgs a) { occlude_body<T, COUNT>(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
+    occlude_body<T, COUNT>(a, xcd_block(blockIdx.x, gridDim.x));
+}
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ d, long n,
                                                             uint64_t* __restrict__ part, const int* status) {
     scan_reduce_body(d, n, part, status, blockIdx.x);
@@ -1269,7 +1306,7 @@ FD_BATCHED(kb_post_pairs, 64, post_pairs_body)
 
 template <typename T>
 __global__ __launch_bounds__(64) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
     occlude_body<T, false>(a, b - map[i]);

@@ -318,7 +318,7 @@ class Context:
 
     def get_profile(self, ops=False):
         ms = (ctypes.c_float * 5)()
-        cnt = (ctypes.c_ulonglong * 6)()
+        cnt = (ctypes.c_ulonglong * 8)()
         _check(lib().fd_get_profile(self.h, ms, 5, cnt if ops else None), "fd_get_profile")
         return list(ms), (list(cnt) if ops else None)
 

@@ -31,10 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "visibility Gbit/s + p50/p99 frame ms, batched-served vs plain, bit-exact masks"
-# operations per counted event of the occlusion arithmetic
-# (Step5_2_shadowCalPerSample.inl:110-223): containment/depth cull, the
-# per-triangle planes + row-range masks, and one light row of three edge masks
-OPS_PER_CAND, OPS_PER_PASS, OPS_PER_ROW = 5, 160, 45
+# algorithmic operations per counted event of the reference's occlusion
+# arithmetic (Step5_2_shadowCalPerSample.inl:110-223): containment/depth
+# cull per candidate, planes (93) + row-range masks (18) per set-up pair,
+# one light row per row the reference visits.  Rows this kernel skips (no
+# set bit possible) still count: the work is the reference algorithm's.
+OPS_PER_CAND, OPS_PER_PLANES, OPS_PER_ROW = 5, 93 + 18, 15
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
@@ -79,7 +81,7 @@ def gpu_plain(pkg, frame, steps, warmup, quick=False):
         src.close()
         dst.close()
         return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=[ms_per_step], stage=[0.0] * 5,
-                    ops=[0] * 6, stats=st.as_dict(), shadow=None, masks=None)
+                    ops=[0] * 8, stats=st.as_dict(), shadow=None, masks=None)
     # latency: one frame at a time, host wall clock around fd_launch
     lat = []
     for _ in range(max(steps, 10)):
@@ -370,8 +372,8 @@ def main():
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
     value = g["bits"] / (g["ms_per_step"] * 1e-3) / 1e9
     occ_ms = max(g["stage"][4], 1e-9)
-    cand, npass, rows, items, wave_it, wave_rows = g["ops"]
-    ops = OPS_PER_CAND * cand + OPS_PER_PASS * npass + OPS_PER_ROW * rows
+    cand, npass, rows, items, wave_it, wave_rows, ref_rows, planes = g["ops"]
+    ops = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * ref_rows
     achieved = ops / (occ_ms * 1e-3) / 1e12
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=FP32_VECTOR_PEAK_TFLOPS, unit="TOP/s",
                     frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), traffic=None,
@@ -380,7 +382,8 @@ def main():
                               work_items=int(items), wave_tri_iterations=int(wave_it),
                               wave_row_iterations=int(wave_rows),
                               lane_util_rows=round(rows / max(1, 64 * wave_rows), 4),
-                              ops_per_event=[OPS_PER_CAND, OPS_PER_PASS, OPS_PER_ROW]))
+                              reference_row_visits=int(ref_rows), planes=int(planes),
+                              ops_per_event=[OPS_PER_CAND, OPS_PER_PLANES, OPS_PER_ROW]))
     out = dict(metric=METRIC, value=round(value, 4), unit="Gbit/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=round(g["ms_per_step"], 4), higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f32", data="synthetic (seeded generators, 4dzb_amd/csrc/scenes.cpp)",

@@ -229,11 +229,17 @@ int fd_server_destroy(fd_server* srv);
 
 /* No counterpart (the reference times the whole FDLaunch with cudaEvents,
  * FD/api.cpp:405-409,574-581): per-stage device times of the next frames
- * (light plane, receiver binning, triangle pairing, tables, occlusion) and
- * occlusion work counters (candidate tests, cull passes, visited rows,
- * work items). */
+ * (light plane, receiver binning, triangle pairing, tables, occlusion) and,
+ * with count_ops, FD_PROFILE_COUNTERS occlusion work counters (a slower
+ * counting variant of the occlusion kernel runs):
+ *   [0] candidate (receiver, triangle) tests   [1] cull passes
+ *   [2] rows visited by the kernel             [3] work items (64 receivers)
+ *   [4] wave triangle iterations               [5] wave row iterations
+ *   [6] rows the reference visits (row-range mask minus full rows)
+ *   [7] pairs whose planes are set up (cull passes not on the triangle). */
+#define FD_PROFILE_COUNTERS 8
 int fd_set_profiling(fd_ctx* ctx, int stage_events, int count_ops);
-int fd_get_profile(fd_ctx* ctx, float* stage_ms, int nstages, unsigned long long* ops4);
+int fd_get_profile(fd_ctx* ctx, float* stage_ms, int nstages, unsigned long long* ops);
 
 /* No counterpart (the reference receives GL objects): device memory and
  * streams for callers without their own HIP allocator.  Copies are

@@ -490,6 +490,48 @@ __device__ __forceinline__ TriClamp tri_clamp(const float4 mn, const float4 mx, 
     return c;
 }
 
+// Visits, in ascending bin order, every bin of the light-plane grid that the
+// bin rectangle [sx, ex) x [sy, ey) of at least one lane covers, calling
+// body(bp, i, j, mine) with a wave-uniform bin bp = j << log2G | i and
+// mine = the lane's rectangle holds it.  Two schedules with the same visiting
+// order (so the per-bin ballot ranking of the callers stays stable):
+//  * the union rectangle of the wave, row by row (cheap per bin; best when
+//    the lanes' rectangles overlap);
+//  * "next covered bin": every lane walks its own rectangle and the wave
+//    takes the smallest next bin of any lane (a wave minimum per bin; best
+//    when the rectangles are scattered, where the union is mostly empty).
+// The cheaper one is picked per wave from the union area and the summed
+// rectangle areas.
+template <typename F>
+__device__ __forceinline__ void for_covered_bins(bool has, int sx, int sy, int ex, int ey, int log2G, F&& body) {
+    const int ux0 = uniform(wave_min_i32(has ? sx : 1 << 20));
+    const int uy0 = uniform(wave_min_i32(has ? sy : 1 << 20));
+    const int ux1 = uniform(wave_max_i32(has ? ex : 0));
+    const int uy1 = uniform(wave_max_i32(has ? ey : 0));
+    if (ux1 <= ux0 || uy1 <= uy0) return;
+    const int unionArea = (ux1 - ux0) * (uy1 - uy0);
+    const int sumArea = uniform(wave_sum_i32(has ? (ex - sx) * (ey - sy) : 0));
+    if (unionArea <= 2 * sumArea) {
+        for (int j = uy0; j < uy1; ++j)
+            for (int i = ux0; i < ux1; ++i)
+                body(j << log2G | i, i, j, has && j >= sy && j < ey && i >= sx && i < ex);
+        return;
+    }
+    constexpr int DONE = 1 << 30;
+    int ci = sx, cj = sy;
+    int cur = has ? (cj << log2G | ci) : DONE;
+    for (;;) {
+        const int bp = uniform(wave_min_i32(cur));
+        if (bp == DONE) break;
+        const bool mine = cur == bp;
+        body(bp, bp & ((1 << log2G) - 1), bp >> log2G, mine);
+        if (mine) {
+            if (++ci == ex) { ci = sx; ++cj; }
+            cur = cj < ey ? (cj << log2G | ci) : DONE;
+        }
+    }
+}
+
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t thist[];  // nb
     const FdState* s = a.st;
@@ -574,25 +616,17 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     if (!act) { c.ok = false; c.sx = c.sy = c.ex_ = c.ey_ = 0; }
     int count = 0, tsx = 6000, tsy = 6000, tex = 0, tey = 0;
     const bool has = c.ok && c.sx < c.ex_ && c.sy < c.ey_;
-    const int ux0 = uniform(wave_min_i32(has ? c.sx : 1 << 20));
-    const int uy0 = uniform(wave_min_i32(has ? c.sy : 1 << 20));
-    const int ux1 = uniform(wave_max_i32(has ? c.ex_ : 0));
-    const int uy1 = uniform(wave_max_i32(has ? c.ey_ : 0));
-    for (int j = uy0; j < uy1; ++j) {
-        for (int i = ux0; i < ux1; ++i) {
-            const int bp = j << log2G | i;
-            const BinStat b = load_bin(a.binStat + 5 * bp);
-            const bool pass = has && j >= c.sy && j < c.ey_ && i >= c.sx && i < c.ex_ &&
-                              bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
-            const uint64_t bal = __ballot(pass);
-            if (pass) {
-                ++count;
-                tsx = min(tsx, i); tsy = min(tsy, j);
-                tex = max(tex, i + 1); tey = max(tey, j + 1);
-            }
-            if (bal && lane_id() == 0) atomicAdd(&thist[bp], (uint32_t)__popcll(bal));
+    for_covered_bins(has, c.sx, c.sy, c.ex_, c.ey_, log2G, [&](int bp, int i, int j, bool mine) {
+        const BinStat b = load_bin(a.binStat + 5 * bp);
+        const bool pass = mine && bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
+        const uint64_t bal = __ballot(pass);
+        if (pass) {
+            ++count;
+            tsx = min(tsx, i); tsy = min(tsy, j);
+            tex = max(tex, i + 1); tey = max(tey, j + 1);
         }
-    }
+        if (bal && lane_id() == 0) atomicAdd(&thist[bp], (uint32_t)__popcll(bal));
+    });
     if (act) {
         a.triCount[t] = count;
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
@@ -620,28 +654,20 @@ __device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk)
         const TriClamp c = tri_clamp(mn, mx, s);
         const int sx = rg.x & 0xffff, sy = rg.x >> 16, ex = rg.y & 0xffff, ey = rg.y >> 16;
         const bool has = act && c.ok && sx < ex && sy < ey;
-        const int ux0 = uniform(wave_min_i32(has ? sx : 1 << 20));
-        const int uy0 = uniform(wave_min_i32(has ? sy : 1 << 20));
-        const int ux1 = uniform(wave_max_i32(has ? ex : 0));
-        const int uy1 = uniform(wave_max_i32(has ? ey : 0));
-        for (int j = uy0; j < uy1; ++j) {
-            for (int i = ux0; i < ux1; ++i) {
-                const int bp = j << log2G | i;
-                const BinStat b = load_bin(a.binStat + 5 * bp);
-                const bool pass = has && j >= sy && j < ey && i >= sx && i < ex &&
-                                  bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
-                const uint64_t bal = __ballot(pass);
-                if (bal) {
-                    int b0 = tcnt[bp];
-                    if (b0 < 0) b0 = (int)a.histT[(size_t)bp * a.nblkT + blk];
-                    if (pass) {
-                        const unsigned long long p = (unsigned long long)(uint32_t)b0 + (unsigned)mbcnt(bal);
-                        if (p < cap) a.sortedTri[p] = t;
-                    }
-                    if (lane_id() == 0) tcnt[bp] = b0 + __popcll(bal);
+        for_covered_bins(has, sx, sy, ex, ey, log2G, [&](int bp, int, int, bool mine) {
+            const BinStat b = load_bin(a.binStat + 5 * bp);
+            const bool pass = mine && bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
+            const uint64_t bal = __ballot(pass);
+            if (bal) {
+                int b0 = tcnt[bp];
+                if (b0 < 0) b0 = (int)a.histT[(size_t)bp * a.nblkT + blk];
+                if (pass) {
+                    const unsigned long long p = (unsigned long long)(uint32_t)b0 + (unsigned)mbcnt(bal);
+                    if (p < cap) a.sortedTri[p] = t;
                 }
+                if (lane_id() == 0) tcnt[bp] = b0 + __popcll(bal);
             }
-        }
+        });
         __syncthreads();
     }
 }
@@ -913,16 +939,17 @@ __device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (
 // |C| above 2^40 (where the products could overflow), are not used; NaN
 // thresholds drop out of fmaxf / fminf.
 template <typename T>
-__device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf) {
+__device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB) {
     const float u20 = 9.5367431640625e-07f;   // 2^-20
     const float tiny = 1.17549435e-38f;       // 2^-126
     const float lim = 1.09951163e+12f;        // 2^40
     float Cp[3];
-    bool ok = true;
+    // mAB = max |A|, |B| (NaN-free maximum); a NaN anywhere only makes the
+    // thresholds NaN, which fmaxf / fminf below drop
+    const bool ok = fmaxf(mAB, fmaxf(fmaxf(fabsf(e[0].C), fabsf(e[1].C)), fabsf(e[2].C))) <= lim;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float a = fabsf(e[k].A), b = fabsf(e[k].B), c = fabsf(e[k].C);
-        ok = ok && a <= lim && b <= lim && c <= lim;
         Cp[k] = e[k].C + __builtin_fmaf(__builtin_fmaf(a + b, Rf, c), u20, tiny);
     }
     float lo = 0.0f, hi = Rf - 1.0f;
@@ -935,11 +962,10 @@ __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf) {
         const float E = __builtin_fmaf(__builtin_fmaf(fabsf(pA) + fabsf(qA), Rf, fabsf(pB) + fabsf(qB)), u20, tiny);
         const float ra = __builtin_amdgcn_rcpf(al);
         const float t0 = -be * ra;
-        const float d = E * fabsf(ra);
+        const float m = __builtin_fmaf(E, fabsf(ra), 9.765625e-4f);   // E/|al| + 2^-10
         const bool opp = (e[i].B > 0.0f && e[j].B < 0.0f) || (e[i].B < 0.0f && e[j].B > 0.0f);
-        const bool use = ok && opp && fabsf(al) >= 9.09494702e-13f && d <= 1024.0f;
+        const bool use = ok && opp && fabsf(al) >= 9.09494702e-13f && m <= 1024.0f;
         const bool lower = (e[i].B > 0.0f) == (al > 0.0f);
-        const float m = d + 9.765625e-4f;
         lo = fmaxf(lo, (use && lower) ? t0 - m : 0.0f);
         hi = fminf(hi, (use && !lower) ? t0 + m : Rf - 1.0f);
     }
@@ -1087,7 +1113,12 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
         // dir = dot(Q - v0, n) (:130-132)
         const float dx = q.q[0] - t.v0[0], dy = q.q[1] - t.v0[1], dz = q.q[2] - t.v0[2];
         const float dir = dot3(dx, dy, dz, t.n[0], t.n[1], t.n[2]);
-        if (fabsf(dir) < eps && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+        // Q on the triangle: the reference skips the pair (:134-135).  The
+        // counting variant tests here, in the reference's order; the timed
+        // variant tests below, only for pairs that can still set a bit (the
+        // outcome is the same: a skipped pair sets nothing).
+        const bool onTri = fabsf(dir) < eps;
+        if (COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
         // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172).  The reference
         // negates t0, t1, L0_Q when dir < 0; negation commutes with every
         // rounding here, so A, B, C are negated instead (identical values up
@@ -1095,7 +1126,6 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
         Edge e[3];
         const float* vb[3] = {t.v1, t.v2, t.v0};
         const float* ev[3] = {t.e01, t.e12, t.e20};
-        bool big = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float ax = vb[k][0] - q.q[0], ay = vb[k][1] - q.q[1], az = vb[k][2] - q.q[2];
@@ -1106,8 +1136,12 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
             float C = dot3(nx, ny, nz, Lx, Ly, Lz);
             if (dir < 0) { A = -A; B = -B; C = -C; }
             e[k].A = A; e[k].B = B; e[k].C = C;
-            big |= !(fabsf(A) <= 1.2676506e+30f && fabsf(B) <= 1.2676506e+30f);
         }
+        // |A|, |B| <= 2^100 for the fast mask form (a NaN dropped by fmaxf is
+        // harmless there: it makes both f1 and f2 false, as in the reference)
+        const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fmaxf(fabsf(e[2].A), fabsf(e[0].B))),
+                                fmaxf(fabsf(e[1].B), fabsf(e[2].B)));
+        const bool big = !(mAB <= 1.2676506e+30f);
         if (COUNT) {
             // the rows the reference visits for this pair: its row-range mask
             // minus the rows already full (general form, before any window)
@@ -1129,9 +1163,10 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
         const bool fast = !__any(big);
         T win = FULL;
         if (fast) {
-            win = row_window<T>(e, Rf);
+            win = row_window<T>(e, Rf, mAB);
             if (!__any(win != (T)0)) continue;   // no lane can gain a bit from this triangle
         }
+        if (!COUNT && onTri && win != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             e[k].nRowN = -(e[k].A * nsub);

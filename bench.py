@@ -41,6 +41,25 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
+PROFILED = {("heightfield-1k", 0): "k_occlude_heightfield-1k.md",
+            ("boxes-1080p", 32): "k_occlude_boxes-1080p-32x32.md"}
+
+
+def profiled_traffic(workload, light_res):
+    """HBM bytes per occlusion launch from the committed rocprofv3 summary of
+    the same workload (FETCH_SIZE and WRITE_SIZE passes, calibrated with a
+    256 MiB copy), or None when no summary covers it."""
+    name = PROFILED.get((workload, light_res))
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    try:
+        blob = open(path).read().split("```json", 1)[1].split("```", 1)[0]
+        return dict(json.loads(blob), source=os.path.relpath(path, ROOT))
+    except (IndexError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -375,8 +394,12 @@ def main():
     cand, npass, rows, items, wave_it, wave_rows, ref_rows, planes = g["ops"]
     ops = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * ref_rows
     achieved = ops / (occ_ms * 1e-3) / 1e12
+    prof = profiled_traffic(args.workload, args.light_res)
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=FP32_VECTOR_PEAK_TFLOPS, unit="TOP/s",
-                    frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), traffic=None,
+                    frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
+                    traffic=prof["traffic_bytes"] if prof else None,
+                    traffic_source=prof["source"] if prof else None,
+                    valu_busy_profiled=prof["valu_busy"] if prof else None,
                     kernel="k_occlude", kernel_ms=round(occ_ms, 4),
                     work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
                               work_items=int(items), wave_tri_iterations=int(wave_it),

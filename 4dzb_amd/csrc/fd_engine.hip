@@ -89,6 +89,7 @@ struct fd_ctx {
     int cull = 1;
     float eps = 1e-6f;
     int gridLog2 = 4;
+    int tileLog2 = 3;     // 8x8 receiver tiles
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
     std::vector<Model> models;
@@ -121,12 +122,14 @@ int hip_status(hipError_t e) {
 
 // Frame arguments that do not depend on workspace placement.
 void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm, int W, int H, const FdLight& L,
-               float eps, int cull, int gridLog2, float* shadow, uint32_t* masks, unsigned long long pairCap) {
+               float eps, int cull, int gridLog2, int tileLog2, float* shadow, uint32_t* masks,
+               unsigned long long pairCap) {
     std::memset(&a, 0, sizeof(a));
     a.L = L;
     a.eps = eps;
     a.cull = cull;
     a.gridLog2 = gridLog2;
+    a.tileLog2 = tileLog2;
     a.verts = md.verts;
     a.vstride = md.vstride;
     a.nverts = md.nverts;
@@ -141,7 +144,8 @@ void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm,
     a.masks = masks;
     a.pairCap = pairCap;
     const int nb = 1 << (2 * gridLog2);
-    a.npos = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+    const int TW = 1 << tileLog2, TH = 64 >> tileLog2;
+    a.npos = ((W + TW - 1) / TW) * ((H + TH - 1) / TH) * 64;
     a.nblkR = cdiv(a.npos, FD_RXB);
     a.nblkT = cdiv(std::max(md.ntris, 1), FD_TRB);
     a.nR = (long)(nb + 1) * a.nblkR;
@@ -197,7 +201,7 @@ int prepare(fd_ctx* c, FdArgs& a) {
         return FD_ERR_STATE;
     if (c->lightDesc.res_w != c->lightDesc.res_h || c->lightDesc.res_w < 1 || c->lightDesc.res_w > 64)
         return FD_ERR_UNSUPPORTED_LIGHT;
-    fill_args(a, c->models[0], c->pos, c->nrm, c->W, c->H, c->light, c->eps, c->cull, c->gridLog2, c->shadow,
+    fill_args(a, c->models[0], c->pos, c->nrm, c->W, c->H, c->light, c->eps, c->cull, c->gridLog2, c->tileLog2, c->shadow,
               c->masks, c->pairCap);
     FdArgs probe = a;
     const size_t bytes = layout(probe, nullptr, c->countOps);
@@ -335,6 +339,20 @@ int fd_set_bin_num(fd_ctx* c, size_t n) {
     while ((size_t(1) << l) < n) ++l;
     if ((size_t(1) << l) != n || l < 3 || l > 6) return -1;  // the reference's -1 for a bad size
     c->gridLog2 = l;
+    return FD_OK;
+}
+
+static int tile_log2(int tw) {
+    for (int l = 3; l <= 6; ++l)
+        if (tw == (1 << l)) return l;
+    return -1;
+}
+
+int fd_set_tile_width(fd_ctx* c, int tw) {
+    if (!c) return FD_ERR_ARG;
+    const int l = tile_log2(tw);
+    if (l < 0) return FD_ERR_ARG;
+    c->tileLog2 = l;
     return FD_OK;
 }
 
@@ -622,11 +640,16 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
             if ((1 << l) != q.bins_per_side || l < 3 || l > 6) { delete b; return FD_ERR_ARG; }
             g = l;
         }
+        int tl = c->tileLog2;
+        if (q.tile_width) {
+            tl = tile_log2(q.tile_width);
+            if (tl < 0) { delete b; return FD_ERR_ARG; }
+        }
         FdLight L;
         light_setup(q.light, L);
         const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
-                  c->eps, c->cull, g, q.d_shadow, q.d_masks, cap);
+                  c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
     }
@@ -685,6 +708,90 @@ extern "C" int fd_batch_destroy(fd_batch* b) {
     b->tables.release();
     delete b;
     return FD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: bin grid and receiver tile shape of one instance class.  Each
+// candidate runs as a one-instance batch (the batched form's own launch
+// sequence) into private output buffers, timed with events on a private
+// stream, and is compared word by word with the default layout's output.
+// ---------------------------------------------------------------------------
+namespace fdk {
+hipError_t count_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned long long* out, hipStream_t st);
+}
+
+extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* cands, int n, int reps, int* best) {
+    if (!c || !inst || !cands || n <= 0 || !best || reps <= 0 || reps > 100) return FD_ERR_ARG;
+    if (c->models.empty()) return FD_ERR_STATE;
+    const int R = inst->light.res_w;
+    if (R < 1 || R > 64 || inst->light.res_h != R) return FD_ERR_UNSUPPORTED_LIGHT;
+    if (inst->width <= 0 || inst->height <= 0 || (long)inst->width * inst->height > (1l << 28)) return FD_ERR_ARG;
+    const size_t WH = (size_t)inst->width * inst->height;
+    const size_t words = WH + WH * (size_t)fd_mask_words(R);   // shadow values, then masks
+    DevBuf ref, out, cnt;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipError_t e = ref.ensure(words * 4);
+    if (e == hipSuccess) e = out.ensure(words * 4);
+    if (e == hipSuccess) e = cnt.ensure(sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    // one layout: returns a status, the median device time and the outputs
+    auto run = [&](int bins, int tw, uint32_t* dst, float* ms) -> int {
+        fd_instance q = *inst;
+        q.bins_per_side = bins;
+        q.tile_width = tw;
+        q.d_shadow = (float*)dst;
+        q.d_masks = dst + WH;
+        fd_batch* b = nullptr;
+        int rc = fd_batch_create(c, &q, 1, &b);
+        if (rc != FD_OK) return rc;
+        hipError_t he = fdk::enqueue_batch(b->plan, st);
+        std::vector<float> t;
+        for (int r = 0; r < reps && he == hipSuccess; ++r) {
+            he = hipEventRecord(ev0, st);
+            if (he == hipSuccess) he = fdk::enqueue_batch(b->plan, st);
+            if (he == hipSuccess) he = hipEventRecord(ev1, st);
+            if (he == hipSuccess) he = hipEventSynchronize(ev1);
+            float x = 0.0f;
+            if (he == hipSuccess) he = hipEventElapsedTime(&x, ev0, ev1);
+            t.push_back(x);
+        }
+        fd_stats s{};
+        if (he == hipSuccess) rc = fd_batch_get_stats(b, 0, &s);
+        else rc = hip_status(he);
+        fd_batch_destroy(b);
+        if (rc != FD_OK) return rc;
+        std::sort(t.begin(), t.end());
+        *ms = t[t.size() / 2];
+        return FD_OK;
+    };
+    int rc = hip_status(e);
+    float refMs = 0.0f;
+    if (rc == FD_OK) rc = run(16, 8, (uint32_t*)ref.p, &refMs);
+    *best = -1;
+    for (int i = 0; i < n && rc == FD_OK; ++i) {
+        float ms = 0.0f;
+        rc = run(cands[i].bins_per_side, cands[i].tile_width, (uint32_t*)out.p, &ms);
+        if (rc == FD_ERR_CAPACITY) {   // this grid needs more pairs than the instance's arena
+            cands[i].ms = -2.0f;
+            rc = FD_OK;
+            continue;
+        }
+        if (rc != FD_OK) break;
+        unsigned long long diff = 0;
+        e = fdk::count_diff((const uint32_t*)ref.p, (const uint32_t*)out.p, words, (unsigned long long*)cnt.p, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&diff, cnt.p, sizeof(diff), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) { rc = hip_status(e); break; }
+        cands[i].ms = diff ? -1.0f : ms;
+        if (!diff && (*best < 0 || ms < cands[*best].ms)) *best = i;
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -13,7 +13,7 @@ struct FdArgs {
     float eps;
     int cull;
     int gridLog2;
-    int pad0;
+    int tileLog2;         // receiver tiles of (1 << tileLog2) x (64 >> tileLog2) pixels, 3..6
     // occluders
     const float* verts;
     int vstride;          // floats between consecutive vertices

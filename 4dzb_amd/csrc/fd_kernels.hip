@@ -246,15 +246,18 @@ __device__ __forceinline__ int rx_key(const RxRect& q, const FdState* s, int nb,
     return k;
 }
 
-// Receivers are traversed in 8x8 pixel tiles (tile-major), so that each
-// 64-lane wave of the sort -- and later of the occlusion -- holds a compact
-// tile rather than a 64-pixel strip.  Position p -> dense id, or -1 for
-// padding outside the view.
-__device__ __forceinline__ int tile_id(int p, int W, int H) {
-    const int W8 = (W + 7) >> 3;
+// Receivers are traversed in pixel tiles of 64 (tile-major): TW x (64/TW)
+// with TW = 1 << twl (8x8 by default), so that each 64-lane wave of the sort
+// -- and later of the occlusion -- holds a compact tile rather than a
+// 64-pixel strip.  The tile shape changes only the order of receivers within
+// a bin, never a result.  Position p -> dense id, or -1 for padding outside
+// the view.
+__device__ __forceinline__ int tile_id(int p, int W, int H, int twl) {
+    const int TW = 1 << twl;
+    const int WT = (W + TW - 1) >> twl;
     const int t = p >> 6, j = p & 63;
-    const int x = (t % W8) * 8 + (j & 7);
-    const int y = (t / W8) * 8 + (j >> 3);
+    const int x = (t % WT) * TW + (j & (TW - 1));
+    const int y = (t / WT) * (64 >> twl) + (j >> twl);
     return (x < W && y < H) ? y * W + x : -1;
 }
 
@@ -269,7 +272,7 @@ __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
     const int base = blk * FD_RXB;
     for (int it = 0; it < FD_RXB / 256; ++it) {
         const int p = base + it * 256 + threadIdx.x;
-        const int i = p < a.npos ? tile_id(p, a.W, a.H) : -1;
+        const int i = p < a.npos ? tile_id(p, a.W, a.H, a.tileLog2) : -1;
         int key = nb;
         RxRect q{};
         const bool in = i >= 0;
@@ -417,7 +420,7 @@ __device__ __forceinline__ void rx_scatter_body(const FdArgs& a, const int blk) 
     const int base = blk * FD_RXB;
     for (int it = 0; it < FD_RXB / 64; ++it) {
         const int pp = base + it * 64 + lane_id();
-        const int i = pp < a.npos ? tile_id(pp, a.W, a.H) : -1;
+        const int i = pp < a.npos ? tile_id(pp, a.W, a.H, a.tileLog2) : -1;
         const bool act = i >= 0;
         const int key = act ? (int)a.rxKey[i] : 0;
         uint64_t eq = __ballot(act);
@@ -1462,6 +1465,23 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
         else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }
+    return hipGetLastError();
+}
+
+// Number of differing 32-bit words of two device buffers (autotuner check).
+__global__ __launch_bounds__(256) void k_count_diff(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                                                    size_t n, unsigned long long* out) {
+    int c = 0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) c += a[i] != b[i];
+    c = wave_sum_i32(c);
+    if (lane_id() == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+hipError_t count_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned long long* out, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(out, 0, sizeof(unsigned long long), st);
+    if (e != hipSuccess) return e;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    k_count_diff<<<(unsigned)std::max<size_t>(blocks, 1), 256, 0, st>>>(a, b, n, out);
     return hipGetLastError();
 }
 

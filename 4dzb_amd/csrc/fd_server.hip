@@ -96,6 +96,7 @@ struct Slot {
 
 struct fd_server {
     fd_server_config cfg{};
+    std::vector<fd_tune_entry> tune;   // per-class layouts (fd_autotune)
     std::vector<Model> scenes;
     DevBuf poolMem;
     Pool pool;
@@ -210,6 +211,10 @@ int fd_server_create(const fd_server_config* cfg, fd_server** out) {
     if (s->cfg.streams <= 0) s->cfg.streams = 4;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 16;
     if (s->cfg.bins_per_side == 0) s->cfg.bins_per_side = 16;
+    if (s->cfg.tile_width == 0) s->cfg.tile_width = 8;
+    if (s->cfg.ntune < 0 || (s->cfg.ntune > 0 && !s->cfg.tune)) { delete s; return FD_ERR_ARG; }
+    s->tune.assign(s->cfg.tune, s->cfg.tune + s->cfg.ntune);
+    s->cfg.tune = nullptr;
     if (!s->cfg.max_pairs) s->cfg.max_pairs = 16ull << 20;
     hipError_t e = s->poolMem.ensure(cfg->hbm_budget_bytes);
     if (e != hipSuccess) { delete s; return hip_status(e); }
@@ -254,10 +259,21 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     q.tSubmit = s->now();
     FdLight L;
     light_setup(r->light, L);
-    int g = 0;
-    while ((1 << g) < s->cfg.bins_per_side) ++g;
+    // layout of this request's class: the tuning table, else the defaults
+    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width;
+    for (const fd_tune_entry& t : s->tune)
+        if ((t.scene < 0 || t.scene == r->scene) && t.width == r->width && t.height == r->height &&
+            t.light_res == r->light.res_w && t.ms >= 0.0f) {
+            bins = t.bins_per_side;
+            tw = t.tile_width;
+            break;
+        }
+    int g = 0, tl = 0;
+    while ((1 << g) < bins) ++g;
+    while ((1 << tl) < tw) ++tl;
+    if ((1 << g) != bins || g < 3 || g > 6 || (1 << tl) != tw || tl < 3 || tl > 6) return FD_ERR_ARG;
     fill_args(q.args, md, (const float4*)r->d_position4, (const float4*)r->d_normal4, r->width,
-              r->height, L, s->cfg.eps, s->cfg.backface_cull, g, r->d_shadow, r->d_masks, s->cfg.max_pairs);
+              r->height, L, s->cfg.eps, s->cfg.backface_cull, g, tl, r->d_shadow, r->d_masks, s->cfg.max_pairs);
     FdArgs probe = q.args;
     q.bytes = layout(probe, nullptr, false);
     if (q.bytes > s->pool.size()) return FD_ERR_CAPACITY;   // can never be admitted

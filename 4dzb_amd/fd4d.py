@@ -54,14 +54,21 @@ class Stats(ctypes.Structure):
 class Instance(ctypes.Structure):
     _fields_ = [("d_position4", ctypes.c_void_p), ("d_normal4", ctypes.c_void_p), ("width", ctypes.c_int),
                 ("height", ctypes.c_int), ("light", AreaLight), ("d_shadow", ctypes.c_void_p),
-                ("d_masks", ctypes.c_void_p), ("bins_per_side", ctypes.c_int), ("pad", ctypes.c_int),
+                ("d_masks", ctypes.c_void_p), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
                 ("max_pairs", ctypes.c_ulonglong)]
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("light_res", ctypes.c_int), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
+                ("ms", ctypes.c_float), ("pad", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("streams", ctypes.c_int), ("max_batch", ctypes.c_int),
                 ("bins_per_side", ctypes.c_int), ("backface_cull", ctypes.c_int), ("eps", ctypes.c_float),
-                ("pad", ctypes.c_int), ("max_pairs", ctypes.c_ulonglong)]
+                ("tile_width", ctypes.c_int), ("max_pairs", ctypes.c_ulonglong), ("tune", ctypes.c_void_p),
+                ("ntune", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -99,6 +106,8 @@ def lib():
             "fd_register_res": ([vp], ip),
             "fd_unregister_res": ([vp], ip),
             "fd_set_bin_num": ([vp, sz], ip),
+            "fd_set_tile_width": ([vp, ip], ip),
+            "fd_autotune": ([vp, ctypes.c_void_p, ctypes.c_void_p, ip, ip, ctypes.POINTER(ip)], ip),
             "fd_set_view": ([vp, ip, ip], ip),
             "fd_set_scenes_esp": ([vp, ctypes.c_float], ip),
             "fd_set_scene_params": ([vp, sz, sz], ip),
@@ -273,6 +282,25 @@ class Context:
         if rc != FD_OK:
             raise FdError(f"fd_set_bin_num({n}) -> {rc}")
 
+    def set_tile_width(self, tw):
+        _check(lib().fd_set_tile_width(self.h, tw), f"fd_set_tile_width({tw})")
+
+    def autotune(self, inst_item, candidates, reps=3):
+        """Autotuner (fd_autotune): inst_item as a Batch item (its outputs are
+        not written); candidates: list of (bins, tile_width).  Returns
+        (best index or -1, [ms per candidate: -1 = not bit-exact, -2 = its
+        pairs do not fit the instance's pair arena])."""
+        it = inst_item
+        inst = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
+                        it["shadow"].ptr, 0, 0, 0, it.get("max_pairs", 0))
+        arr = (TuneEntry * len(candidates))()
+        for k, (b, t) in enumerate(candidates):
+            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, 0)
+        best = ctypes.c_int(-1)
+        _check(lib().fd_autotune(self.h, ctypes.byref(inst), ctypes.cast(arr, ctypes.c_void_p), len(candidates),
+                                 reps, ctypes.byref(best)), "fd_autotune")
+        return best.value, [arr[k].ms for k in range(len(candidates))]
+
     def set_pair_capacity(self, n):
         _check(lib().fd_set_pair_capacity(self.h, n), "fd_set_pair_capacity")
 
@@ -353,6 +381,7 @@ class FrameRunner:
         c.set_backface_cull(frame.backface_cull)
         c.set_scenes_esp(frame.eps)
         c.set_bin_num(1 << frame.grid_log2)
+        c.set_tile_width(frame.tile_width)
         if pair_capacity:
             c.set_pair_capacity(pair_capacity)
         c.add_model(self.verts, sc.nverts, sc.verts.shape[1] * 4, self.idx, sc.ntris)
@@ -385,12 +414,13 @@ class Batch:
 
     def __init__(self, ctx, items):
         """items: list of dicts with keys pos, nrm (DeviceBuffer), w, h, light
-        (LightDesc), shadow, masks (DeviceBuffer or None), bins (0 = ctx)."""
+        (LightDesc), shadow, masks (DeviceBuffer or None), bins and tile
+        (0 = the context's setting), max_pairs."""
         arr = (Instance * len(items))()
         for k, it in enumerate(items):
             arr[k] = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
                               it["shadow"].ptr, it["masks"].ptr if it.get("masks") is not None else 0,
-                              it.get("bins", 0), 0, it.get("max_pairs", 0))
+                              it.get("bins", 0), it.get("tile", 0), it.get("max_pairs", 0))
         h = ctypes.c_void_p()
         _check(lib().fd_batch_create(ctx.h, ctypes.cast(arr, ctypes.c_void_p), len(items), ctypes.byref(h)),
                "fd_batch_create")
@@ -424,8 +454,17 @@ class Batch:
 class Server:
     """Serving form: resident scenes, continuous batching over streams."""
 
-    def __init__(self, budget_bytes, streams=4, max_batch=16, bins=16, cull=1, eps=1e-3, max_pairs=0):
-        cfg = ServerConfig(budget_bytes, streams, max_batch, bins, cull, eps, 0, max_pairs)
+    def __init__(self, budget_bytes, streams=4, max_batch=16, bins=16, cull=1, eps=1e-3, max_pairs=0, tile=8,
+                 tune=None):
+        """tune: optional list of dicts (scene, w, h, res, bins, tile) -- the
+        per-class layouts picked by the autotuner."""
+        tune = tune or []
+        self._tune = (TuneEntry * max(1, len(tune)))()
+        for k, t in enumerate(tune):
+            self._tune[k] = TuneEntry(t.get("scene", -1), t["w"], t["h"], t["res"], t["bins"], t["tile"],
+                                      float(t.get("ms", 0.0)), 0)
+        cfg = ServerConfig(budget_bytes, streams, max_batch, bins, cull, eps, tile, max_pairs,
+                           ctypes.cast(self._tune, ctypes.c_void_p).value if tune else None, len(tune), 0)
         h = ctypes.c_void_p()
         _check(lib().fd_server_create(ctypes.cast(ctypes.pointer(cfg), ctypes.c_void_p), ctypes.byref(h)),
                "fd_server_create")

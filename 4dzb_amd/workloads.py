@@ -116,6 +116,7 @@ class Frame:
     eps: float = 1e-3
     backface_cull: int = 1
     grid_log2: int = 4
+    tile_width: int = 8
 
 
 def receivers_plane(w, h, half=4.0):

@@ -178,6 +178,14 @@ def gpu_batched(pkg, reps=3):
         plain(False)
         t.append(time.perf_counter() - a)
     plain_async = min(t)
+    # autotuner: per class (receiver size, light grid) the fastest bit-exact
+    # layout (bins x tile width), measured on this device (untimed)
+    at = pkg.autotune
+    t_tune = time.perf_counter()
+    cache = at.tune_classes(ctx, 0, items)
+    t_tune = time.perf_counter() - t_tune
+    for it in items:
+        it["bins"], it["tile"] = at.layout(cache, 0, it["w"], it["h"], it["light"].res_w)
     # batched, graph-captured
     b = fd.Batch(ctx, items)
     b.capture(strm.handle)
@@ -202,7 +210,10 @@ def gpu_batched(pkg, reps=3):
                 batched_graph_ms=round(batched * 1e3, 2),
                 plain_sync_gbps=round(gb / plain_sync, 3), plain_async_gbps=round(gb / plain_async, 3),
                 batched_gbps=round(gb / batched, 3), speedup_vs_plain_sync=round(plain_sync / batched, 3),
-                speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact))
+                speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact),
+                autotune=dict(seconds=round(t_tune, 2), classes={k: dict(bins=v["bins"], tile=v["tile"],
+                                                                        ms=v["ms"], default_ms=v["default_ms"])
+                                                               for k, v in sorted(cache.items())}))
 
 
 def _digest(bufs):
@@ -260,6 +271,15 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
         svc.append((time.perf_counter() - a) * 1e3)
         assert rc == 0
         ref.append(_digest(outs[k]))
+    # autotuner: one representative request per class (scene, size, grid)
+    at = pkg.autotune
+    t_tune = time.perf_counter()
+    cache = {}
+    for k, q in enumerate(reqs):
+        r, pos, nrm = rx[(q["w"], q["h"])]
+        item = dict(pos=pos, nrm=nrm, w=q["w"], h=q["h"], light=lights[k], shadow=outs[k][0], max_pairs=64 << 20)
+        at.tune_classes(ctxs[q["scene"]], q["scene"], [item], cache, reps=2)
+    t_tune = time.perf_counter() - t_tune
     for c in ctxs:
         c.close()
     arr = [q["t"] * 1e3 for q in reqs]
@@ -270,7 +290,9 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
         lat_plain.append(t_free - a)
     plain_span = t_free - arr[0]
     # served
-    srv = fd.Server(budget_bytes=budget_gb << 30, streams=4, max_batch=16, eps=1e-3)
+    # same pair arena per request as the plain path's contexts (64M pairs)
+    srv = fd.Server(budget_bytes=budget_gb << 30, streams=4, max_batch=16, eps=1e-3, max_pairs=64 << 20,
+                    tune=at.server_table(cache))
     for (v, i), sc in zip(dev_scenes, scenes):
         srv.add_scene(v, sc.nverts, 16, i, sc.ntris)
     for o in outs:
@@ -290,7 +312,9 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
     srv.close()
     lat = [c.t_done_ms - c.t_submit_ms for c in done]
     span = max(c.t_done_ms for c in done) - min(c.t_submit_ms for c in done)
-    exact = all(_digest(outs[k]) == ref[k] and done[k].status == 0 for k in range(len(reqs)))
+    failed = sum(1 for c in done if c.status != 0)
+    mismatched = sum(1 for k in range(len(reqs)) if _digest(outs[k]) != ref[k])
+    exact = failed == 0 and mismatched == 0 and len(done) == len(reqs)
     gb = bits / 1e9
     return dict(workload=f"serve-poisson ({nreq} requests at {rate:.0f} req/s; scenes 100k/300k/1M/2M triangles; "
                 "receivers from {256^2, 512^2, 1024^2, 1920x1080}; light grids 16^2 / 32^2)",
@@ -302,7 +326,9 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
                 plain_p50_ms=round(pctl(lat_plain, 50), 2), plain_p99_ms=round(pctl(lat_plain, 99), 2),
                 plain_frame_ms_mean=round(float(np.mean(svc)), 2),
                 mean_batch=round(float(np.mean([c.batch_size for c in done])), 2),
-                masks_bit_exact_vs_plain=bool(exact))
+                masks_bit_exact_vs_plain=bool(exact), failed_requests=failed, mismatched_requests=mismatched,
+                autotune=dict(seconds=round(t_tune, 2), classes=len(cache),
+                              non_default=sum(1 for v in cache.values() if (v["bins"], v["tile"]) != (16, 8))))
 
 
 def gpu_target_config(pkg, steps=10):

@@ -118,6 +118,11 @@ int fd_unregister_res(fd_ctx* ctx);
  * (default 16 = FD_MAX_LP_GRID_SIZE, constants.h:13,45). */
 int fd_set_bin_num(fd_ctx* ctx, size_t bins_per_side);
 
+/* No counterpart (a layout knob of this implementation): receivers are
+ * walked in tiles of tile_width x (64 / tile_width) pixels, tile_width in
+ * {8, 16, 32, 64} (default 8).  Changes speed only, never a result. */
+int fd_set_tile_width(fd_ctx* ctx, int tile_width);
+
 /* FD::FDsetView, FD/api.cpp:205-237. */
 int fd_set_view(fd_ctx* ctx, int width, int height);
 
@@ -162,7 +167,7 @@ typedef struct fd_instance {
     float* d_shadow;            /* width*height floats */
     uint32_t* d_masks;          /* width*height*fd_mask_words(res) words, or NULL */
     int bins_per_side;          /* 0 = the context's setting */
-    int pad;
+    int tile_width;             /* 0 = the context's setting (fd_set_tile_width) */
     unsigned long long max_pairs; /* pair arena of this instance; 0 = 16M */
 } fd_instance;
 typedef struct fd_batch fd_batch;
@@ -183,6 +188,16 @@ int fd_batch_destroy(fd_batch* batch);
  * buffers named in a request must stay valid until its completion is
  * polled. */
 typedef struct fd_server fd_server;
+/* One tuned layout: requests of this scene (-1 = any), receiver size and
+ * light grid run with these bins and tile width (see fd_autotune). */
+typedef struct fd_tune_entry {
+    int scene;
+    int width, height, light_res;
+    int bins_per_side;
+    int tile_width;
+    float ms;                 /* measured frame time; < 0: not bit-exact */
+    int pad;
+} fd_tune_entry;
 typedef struct fd_server_config {
     size_t hbm_budget_bytes;  /* workspace pool (>= 64 MiB) */
     int streams;              /* batches in flight (default 4) */
@@ -190,8 +205,11 @@ typedef struct fd_server_config {
     int bins_per_side;        /* default 16 */
     int backface_cull;
     float eps;
-    int pad;
+    int tile_width;           /* default 8 */
     unsigned long long max_pairs; /* per request; 0 = 16M */
+    const fd_tune_entry* tune;  /* optional per-class layouts (copied at creation) */
+    int ntune;
+    int pad;
 } fd_server_config;
 typedef struct fd_request {
     int scene;                /* id from fd_server_add_scene */
@@ -226,6 +244,17 @@ int fd_server_poll(fd_server* srv, fd_completion* out, int max);
 int fd_server_drain(fd_server* srv, int timeout_ms);
 double fd_server_now_ms(fd_server* srv);
 int fd_server_destroy(fd_server* srv);
+
+/* No counterpart: the autotuner.  Runs `inst` over the context's scene once
+ * per candidate layout (cands[i].bins_per_side, cands[i].tile_width), timing
+ * `reps` launches on the device, and compares every output bit with the
+ * default layout (16x16 bins, 8-wide tiles).  cands[i].ms receives the median
+ * frame time, -1 when the candidate's shadow values or masks differ (the bin
+ * grid is observable at exact light-plane-edge ties), or -2 when its pairs do
+ * not fit the instance's pair arena (max_pairs).  *best receives the fastest
+ * bit-exact candidate.  The instance's own output buffers are not
+ * written.  Synchronises the device. */
+int fd_autotune(fd_ctx* ctx, const fd_instance* inst, fd_tune_entry* cands, int n, int reps, int* best);
 
 /* No counterpart (the reference times the whole FDLaunch with cudaEvents,
  * FD/api.cpp:405-409,574-581): per-stage device times of the next frames

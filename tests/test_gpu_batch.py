@@ -79,3 +79,30 @@ def test_batch_instance_against_oracle(fd, wl, oracle):
         assert np.array_equal(it["masks"].download(np.uint32), o.masks), k
     b.close()
     ctx.close()
+
+
+def test_tile_widths_and_autotune(fd, wl, oracle):
+    """Every receiver tile shape gives the oracle's bits; the autotuner
+    reports a time for every bit-exact candidate and picks one of them."""
+    frame = wl.plane_256(ntri=3000, res=96, light_res=12)
+    o = oracle.run(frame)
+    for tw in (8, 16, 32, 64):
+        frame.tile_width = tw
+        print(f"[tile] width {tw}", flush=True)
+        fr = fd.FrameRunner(frame)
+        rc, st = fr.run()
+        sh, mk = fr.outputs()
+        assert rc == 0 and st.exit_code == o.exit
+        assert np.array_equal(sh.view(np.uint32), np.asarray(o.shadow, np.float32).view(np.uint32))
+        assert np.array_equal(mk, o.masks.reshape(mk.shape))
+        fr.close()
+    frame.tile_width = 8
+    fr = fd.FrameRunner(frame)
+    item = dict(pos=fr.pos, nrm=fr.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=fr.shadow)
+    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64)]
+    best, ms = fr.ctx.autotune(item, cands, reps=2)
+    assert 0 <= best < len(cands) and ms[best] > 0
+    assert ms[1] > 0   # the reference layout itself is always exact
+    assert all(m in (-1, -2) or m > 0 for m in ms)
+    assert ms[best] == min(m for m in ms if m > 0)
+    fr.close()

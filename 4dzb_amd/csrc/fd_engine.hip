@@ -152,6 +152,13 @@ void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm,
     a.nT = (long)nb * a.nblkT;
     a.nbR = fdk::reduce_blocks(a.WH);
     a.nbV = fdk::reduce_blocks(a.nverts);
+    // small frames (up to 2048 work items: two waves per SIMD) leave most of
+    // the GPU idle with one wave per 64 receivers, each walking its bin's
+    // whole triangle list: split the lists to about 16k work items.  Larger
+    // frames fill the GPU already, and splitting would only cost the merge and
+    // the early exit on full rows.
+    a.chunkSlots = cdiv(a.WH, 64) + nb + 1;
+    a.segs = (md.ntris > 0 && a.chunkSlots <= 2048) ? std::max(1, std::min(16, 16384 / a.chunkSlots)) : 1;
 }
 
 // Place the frame's workspace at `base` (nullptr: size only); returns bytes.
@@ -575,8 +582,14 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
     P.occ[0] = dArgs + n;
     P.occ[1] = dArgs + n + P.nocc[0];
     for (int m = 0; m < FD_NMAPS; ++m) {
-        const bool occ = (m == FD_MAP_OCC32 || m == FD_MAP_OCC64);
+        const bool occ = (m == FD_MAP_OCC32 || m == FD_MAP_OCC64 || m == FD_MAP_ZERO || m == FD_MAP_FIN);
         const std::vector<int>* order = occ ? &occIdx[m == FD_MAP_OCC64 ? 1 : 0] : nullptr;
+        std::vector<int> all;   // zero / finish maps: every instance, in occlusion-table order
+        if (m == FD_MAP_ZERO || m == FD_MAP_FIN) {
+            all = occIdx[0];
+            all.insert(all.end(), occIdx[1].begin(), occIdx[1].end());
+            order = &all;
+        }
         const int cnt = occ ? (int)order->size() : n;
         int* mp = hMaps + (size_t)m * (n + 1);
         P.map[m] = dMaps + (size_t)m * (n + 1);
@@ -594,7 +607,9 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
                 case FD_MAP_SCANR: blocks = fdk::scan_chunks(a.nR); break;
                 case FD_MAP_TRI: blocks = a.nblkT; break;
                 case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
-                default: blocks = cdiv(a.WH, 64) + nb + 1; break;
+                case FD_MAP_ZERO: blocks = a.segs > 1 ? fdk::zero_blocks(a) : 0; break;
+                case FD_MAP_FIN: blocks = a.segs > 1 ? a.chunkSlots : 0; break;
+                default: blocks = a.chunkSlots * a.segs; break;
             }
             acc += blocks;
             mp[q + 1] = acc;

@@ -54,6 +54,12 @@ struct FdArgs {
     int* chunkStart;      // nb + 2
     uint64_t* scanPart;   // scan partial sums
     unsigned long long* opCount;  // optional occlusion counters (FD_NOPS), or null
+    // occlusion work split: chunkSlots = receiver chunk slots (64 each, an upper
+    // bound of the work items); each bin's triangle list is cut into segs
+    // segments, one work item per (chunk, segment); with segs > 1 the partial
+    // masks merge by atomic OR into zeroed masks and a finishing pass counts them
+    int chunkSlots;
+    int segs;
 };
 
 // Occlusion counters, in the order documented at fd_get_profile.
@@ -64,7 +70,7 @@ struct FdArgs {
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
 enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,
-             FD_MAP_OCC64, FD_NMAPS };
+             FD_MAP_OCC64, FD_MAP_ZERO, FD_MAP_FIN, FD_NMAPS };
 
 struct FdBatchPlan {
     const FdArgs* inst;       // device: instance table (n)
@@ -83,4 +89,5 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st);
 long scan_partials_needed(long n);
 int reduce_blocks(long n);
 int scan_chunks(long n);
+int zero_blocks(const FdArgs& a);   // blocks of the mask-zeroing pass
 }  // namespace fdk

@@ -926,14 +926,16 @@ __device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (
 // implies A r + B c + C >= -tau with tau = 4u (|A| R + |B| R + |C|)
 // (u = 2^-24: the rounding of A*r, of + C, of the quotient and of B*(R-1);
 // the set bit means c < q (prefix), c >= q (suffix) or both end columns
-// pass).  The code uses tau' = 2^-20 (...) + 2^-126 and C' = fl(C + tau').
+// pass).  The code uses one tau' = 2^-20 (2 M R + Mc) + 2^-126 for the three
+// edges (M = max |A|, |B|; Mc = max |C|) and C' = fl(C + tau').
 // A prefix edge m and a suffix edge p meet in some real column only if
 //   |B_m| (A_p r + C'_p) + B_p (A_m r + C'_m) >= 0,
 // which for a pair (i, j) of opposite-signed B is  s G(r) >= 0  with
 // G(r) = (B_i A_j - B_j A_i) r + (B_i C'_j - B_j C'_i) = al r + be, s = sign(B_i).
-// al and be are computed in float; E = 2^-20 (Sa R + Sb) + 2^-126 bounds
-// their rounding (Sa, Sb: sums of the absolute products), so every row with a
-// non-empty mask has  s (al r + be) + E >= 0, i.e.
+// al and be are computed in float; their rounding is at most 3u (Sa R + Sb)
+// (Sa, Sb: sums of the absolute products, Sa <= 2 M^2, Sb <= 2 M Mc'), and
+// E = 2^-19 M (M R + Mc') + 2^-126 bounds it, so every row with a non-empty
+// mask has  s (al r + be) + E >= 0, i.e.
 //   r >= t0 - E/|al|  (s al > 0)   or   r <= t0 + E/|al|  (s al < 0),  t0 = -be/al.
 // t0 and E/|al| come from v_rcp_f32 (2^-22 relative); a margin of 2^-10
 // covers that while |t0| <= 2^11, and a pair is used only when
@@ -946,15 +948,13 @@ __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB)
     const float u20 = 9.5367431640625e-07f;   // 2^-20
     const float tiny = 1.17549435e-38f;       // 2^-126
     const float lim = 1.09951163e+12f;        // 2^40
-    float Cp[3];
     // mAB = max |A|, |B| (NaN-free maximum); a NaN anywhere only makes the
     // thresholds NaN, which fmaxf / fminf below drop
-    const bool ok = fmaxf(mAB, fmaxf(fmaxf(fabsf(e[0].C), fabsf(e[1].C)), fabsf(e[2].C))) <= lim;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float a = fabsf(e[k].A), b = fabsf(e[k].B), c = fabsf(e[k].C);
-        Cp[k] = e[k].C + __builtin_fmaf(__builtin_fmaf(a + b, Rf, c), u20, tiny);
-    }
+    const float mC = fmaxf(fmaxf(fabsf(e[0].C), fabsf(e[1].C)), fabsf(e[2].C));
+    const bool ok = fmaxf(mAB, mC) <= lim;
+    const float tau = __builtin_fmaf(__builtin_fmaf(mAB, 2.0f * Rf, mC), u20, tiny);
+    const float Cp[3] = {e[0].C + tau, e[1].C + tau, e[2].C + tau};
+    const float E = __builtin_fmaf(__builtin_fmaf(mAB, Rf, mC + tau), mAB * 1.9073486328125e-06f, tiny);
     float lo = 0.0f, hi = Rf - 1.0f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -962,7 +962,6 @@ __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB)
         const float pA = e[i].B * e[j].A, qA = e[j].B * e[i].A;
         const float pB = e[i].B * Cp[j], qB = e[j].B * Cp[i];
         const float al = pA - qA, be = pB - qB;
-        const float E = __builtin_fmaf(__builtin_fmaf(fabsf(pA) + fabsf(qA), Rf, fabsf(pB) + fabsf(qB)), u20, tiny);
         const float ra = __builtin_amdgcn_rcpf(al);
         const float t0 = -be * ra;
         const float m = __builtin_fmaf(E, fabsf(ra), 9.765625e-4f);   // E/|al| + 2^-10
@@ -1005,8 +1004,31 @@ __device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds
     }
 }
 
+// Merge of a segment's partial masks: atomic OR of the non-zero words into
+// the (zeroed) output; same word order as store_masks.
+__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, const int* ids, int n, int R, int wpr) {
+    const int per = R * wpr;
+    const int total = n * per;
+    for (int j = lane_id(); j < total; j += 64) {
+        const int l = j / per, w = j - l * per;
+        const int r = w / wpr, h = w - r * wpr;
+        const uint32_t v = lds[(h * R + r) * 64 + l];
+        if (v) atomicOr(a.masks + (size_t)ids[l] * per + w, v);
+    }
+}
+
+// Work item -> (bin, first sorted position): binary search over the chunk prefix.
+__device__ __forceinline__ int chunk_bin(const FdArgs& a, int c, int nb) {
+    int lo = 0, hi = nb + 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.chunkStart[mid] <= c) lo = mid; else hi = mid;
+    }
+    return uniform(lo);
+}
+
 template <typename T, bool COUNT>
-__device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
+__device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 ids
     const FdState* s = a.st;
     const int status = s->status;
@@ -1015,9 +1037,14 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
     const int lane = lane_id();
     int* ids = (int*)(lds + wpr * R * 64);
     const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
-    const int c = blk;
+    // work item = (chunk c, triangle segment seg); segment-major so that
+    // neighbouring work items share a stretch of the same triangle list
+    const int seg = blkIn / a.chunkSlots;
+    const int c = blkIn - seg * a.chunkSlots;
+    const int S = a.segs;
 
     if (status != FD_EXIT_OK) {
+        if (seg != 0) return;
         // early exit of the frame: every receiver shadow 0, except back-facing
         // receivers once the back-face stage has run (FD/api.cpp:489,
         // Step2_1_samplerRectangleAABB.inl:97)
@@ -1042,13 +1069,20 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
     }
     if (c >= s->totalChunks) return;
     const int nb = s->nb;
-    // work item -> bin (binary search over the chunk prefix)
-    int lo = 0, hi = nb + 1;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (a.chunkStart[mid] <= c) lo = mid; else hi = mid;
+    const int bin = chunk_bin(a, c, nb);
+    if (bin == nb && seg != 0) return;
+    // this work item's stretch of the bin's triangle list (bins 0..nb-1)
+    int ts = 0, te = 0;
+    if (bin != nb) {
+        ts = a.triStart[bin];
+        te = a.triEnd[bin];
+        if (S > 1) {
+            const int per = (te - ts + S - 1) / S;
+            ts += seg * per;
+            te = min(te, ts + per);
+            if (ts >= te) return;   // nothing to add; the finishing pass counts the merged masks
+        }
     }
-    const int bin = uniform(lo);
     const int s0 = a.rxStart[bin] + (c - a.chunkStart[bin]) * 64;
     const int n = min(64, a.rxEnd[bin] - s0);
     const bool act = lane < n;
@@ -1083,11 +1117,17 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
     const float Lx = a.L.ur[0] - q.q[0], Ly = a.L.ur[1] - q.q[1], Lz = a.L.ur[2] - q.q[2];  // L0 - Q
     const float xd0 = a.L.xd[0], xd1 = a.L.xd[1], xd2 = a.L.xd[2];
     const float yd0 = a.L.yd[0], yd1 = a.L.yd[1], yd2 = a.L.yd[2];
+    // light axes along x and y (x_delt = (c, 0, 0), y_delt = (0, c', 0), zeros
+    // of either sign): then dot(n, x_delt) = n.x * c exactly, up to the sign
+    // of a zero result, whenever n is finite (the other two products are
+    // zeros).  No later step can tell +0 from -0 (comparisons with 0, the
+    // exact-division fallback maps both infinities of a zero divisor to R, and
+    // the row window only uses magnitudes).
+    const bool axial = xd1 == 0.0f && xd2 == 0.0f && yd0 == 0.0f && yd2 == 0.0f;
     const float nsub = (float)(unsigned)(R - 1);
     const float Rf = (float)R;
     const float eps = a.eps;
     T full = 0;
-    const int ts = a.triStart[bin], te = a.triEnd[bin];
     unsigned long long nCand = 0, nPass = 0, nRows = 0, nWaveIt = 0, nWaveRows = 0, nRefRows = 0, nPlanes = 0;
     // read-only for this kernel: the constant address space lets the uniform
     // per-triangle loads go through the scalar unit (s_load)
@@ -1129,16 +1169,36 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
         Edge e[3];
         const float* vb[3] = {t.v1, t.v2, t.v0};
         const float* ev[3] = {t.e01, t.e12, t.e20};
+        float nv[3][3];
+        bool finiteC = true;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float ax = vb[k][0] - q.q[0], ay = vb[k][1] - q.q[1], az = vb[k][2] - q.q[2];
             const float bx = ev[k][0], by = ev[k][1], bz = ev[k][2];
-            const float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-            float A = dot3(nx, ny, nz, xd0, xd1, xd2);
-            float B = dot3(nx, ny, nz, yd0, yd1, yd2);
-            float C = dot3(nx, ny, nz, Lx, Ly, Lz);
-            if (dir < 0) { A = -A; B = -B; C = -C; }
-            e[k].A = A; e[k].B = B; e[k].C = C;
+            nv[k][0] = ay * bz - az * by;
+            nv[k][1] = az * bx - ax * bz;
+            nv[k][2] = ax * by - ay * bx;
+            e[k].C = dot3(nv[k][0], nv[k][1], nv[k][2], Lx, Ly, Lz);
+            finiteC = finiteC && __builtin_isfinite(e[k].C);
+        }
+        // a finite C = dot(n, L0 - Q) implies a finite n (an infinite
+        // component makes a product infinite or NaN)
+        if (axial && !__any(!finiteC)) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[k].A = nv[k][0] * xd0;
+                e[k].B = nv[k][1] * yd1;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[k].A = dot3(nv[k][0], nv[k][1], nv[k][2], xd0, xd1, xd2);
+                e[k].B = dot3(nv[k][0], nv[k][1], nv[k][2], yd0, yd1, yd2);
+            }
+        }
+        if (dir < 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { e[k].A = -e[k].A; e[k].B = -e[k].B; e[k].C = -e[k].C; }
         }
         // |A|, |B| <= 2^100 for the fast mask form (a NaN dropped by fmaxf is
         // harmless there: it makes both f1 and f2 false, as in the reference)
@@ -1232,13 +1292,15 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
             rows([&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
         }
     }
-    // fraction = popcount / (W*H) (:226-231)
-    int cnt = 0;
-    for (int r = 0; r < R; ++r) {
-        cnt += __popc(lds[r * 64 + lane]);
-        if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
+    // fraction = popcount / (W*H) (:226-231); split lists: after the merge
+    if (S == 1) {
+        int cnt = 0;
+        for (int r = 0; r < R; ++r) {
+            cnt += __popc(lds[r * 64 + lane]);
+            if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
+        }
+        if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
     }
-    if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
     if (COUNT) {
         // per-lane counts: nCand is wave-wide already
         unsigned long long p = nPass, r = nRows, rr = nRefRows, pl = nPlanes;
@@ -1262,7 +1324,39 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blk) {
         }
     }
     __syncthreads();
-    store_masks<T>(a, lds, ids, n, R, wpr);
+    if (S == 1) store_masks<T>(a, lds, ids, n, R, wpr);
+    else or_masks(a, lds, ids, n, R, wpr);
+}
+
+// Split lists only: zeroes the mask output before the occlusion merges into
+// it (a fill kernel, so that the sequence stays graph-capturable).
+constexpr int ZERO_WORDS = 4096;   // words per block
+__device__ __forceinline__ void zero_masks_body(const FdArgs& a, const int blk) {
+    const size_t n = (size_t)a.WH * a.L.res * a.L.wpr;
+    const size_t base = (size_t)blk * ZERO_WORDS;
+    for (int k = threadIdx.x; k < ZERO_WORDS; k += blockDim.x)
+        if (base + k < n) a.masks[base + k] = 0u;
+}
+
+// Split lists only: shadow value of every receiver of a real bin from its
+// merged mask (popcount / samples, :226-231).
+__device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c) {
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK || c >= s->totalChunks) return;
+    const int nb = s->nb;
+    const int bin = chunk_bin(a, c, nb);
+    if (bin == nb) return;
+    const int s0 = a.rxStart[bin] + (c - a.chunkStart[bin]) * 64;
+    const int n = min(64, a.rxEnd[bin] - s0);
+    const int lane = lane_id();
+    if (lane >= n) return;
+    const int id = a.sortedId[s0 + lane];
+    if ((unsigned)id >= (unsigned)a.WH) return;
+    const int per = a.L.res * a.L.wpr;
+    const uint32_t* m = a.masks + (size_t)id * per;
+    int cnt = 0;
+    for (int w = 0; w < per; ++w) cnt += __popc(m[w]);
+    a.shadow[id] = (float)cnt / (float)a.L.resNum;
 }
 
 // ==========================================================================
@@ -1294,6 +1388,9 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
     const int j = b % (8 * G);
     return b - j + (j & 7) * G + (j >> 3);
 }
+
+__global__ __launch_bounds__(256) void k_zero_masks(FdArgs a) { zero_masks_body(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_occlude_finish(FdArgs a) { occlude_finish_body(a, blockIdx.x); }
 
 template <typename T, bool COUNT>
 __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
@@ -1348,6 +1445,18 @@ __global__ __launch_bounds__(64) void kb_occlude(const FdArgs* __restrict__ inst
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
     occlude_body<T, false>(a, b - map[i]);
+}
+__global__ __launch_bounds__(256) void kb_zero_masks(const FdArgs* __restrict__ inst, const int* __restrict__ map,
+                                                     int n) {
+    const int b = blockIdx.x;
+    const int i = inst_of(map, n, b);
+    zero_masks_body(inst[i], b - map[i]);
+}
+__global__ __launch_bounds__(64) void kb_occlude_finish(const FdArgs* __restrict__ inst, const int* __restrict__ map,
+                                                        int n) {
+    const int b = blockIdx.x;
+    const int i = inst_of(map, n, b);
+    occlude_finish_body(inst[i], b - map[i]);
 }
 
 // which = 0: receiver key histogram, 1: pair histogram
@@ -1436,6 +1545,7 @@ static hipError_t scan_u32(uint32_t* d, long n, uint64_t* part, unsigned long lo
 long scan_partials_needed(long n) { return cdiv(n, SCAN_CHUNK); }
 int reduce_blocks(long n) { return cdiv(n, RED_BLOCK * RED_ITEMS); }
 int scan_chunks(long n) { return cdiv(n, SCAN_CHUNK); }
+int zero_blocks(const FdArgs& a) { return (int)cdiv((long)a.WH * a.L.res * a.L.wpr, ZERO_WORDS); }
 
 hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     const int n = p.n;
@@ -1459,12 +1569,14 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     kb_post_pairs<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.anyTris) kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
     for (int c = 0; c < 2; ++c) {
         if (p.nocc[c] == 0) continue;
         const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t) + 64 * sizeof(int);
         if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
         else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }
+    if (p.total[FD_MAP_FIN] > 0) kb_occlude_finish<<<L(FD_MAP_FIN), 64, 0, st>>>(p.occ[0], p.map[FD_MAP_FIN], n);
     return hipGetLastError();
 }
 
@@ -1529,10 +1641,12 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     mark(3);
     k_tables<<<1, 1024, 0, st>>>(a);
     mark(4);
-    // occlusion (or the early-exit fill)
-    const int chunks = cdiv(a.WH, 64) + nb + 1;
+    // occlusion (or the early-exit fill); split lists zero the masks first
+    // and count them after the merge
+    const int chunks = a.chunkSlots * a.segs;
     const int wpr = a.L.wpr;
     const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * sizeof(int);
+    if (a.segs > 1) k_zero_masks<<<zero_blocks(a), 256, 0, st>>>(a);
     if (a.opCount) {
         if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64, shm, st>>>(a);
         else k_occlude<uint64_t, true><<<chunks, 64, shm, st>>>(a);
@@ -1540,6 +1654,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         if (wpr == 1) k_occlude<uint32_t, false><<<chunks, 64, shm, st>>>(a);
         else k_occlude<uint64_t, false><<<chunks, 64, shm, st>>>(a);
     }
+    if (a.segs > 1) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
     mark(5);
     return hipGetLastError();
 }

@@ -571,10 +571,17 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
     const size_t argBytes = (sizeof(FdArgs) * (size_t)(2 * n) + 255) & ~(size_t)255;
     image.assign(argBytes + (size_t)FD_NMAPS * (n + 1) * sizeof(int), 0);
     FdArgs* hArgs = (FdArgs*)image.data();
-    for (int i = 0; i < n; ++i) hArgs[i] = args[i];
+    // a batch that fills the GPU with whole-list work items does not split the
+    // triangle lists of its small instances (the merge would only cost)
+    long slots = 0;
+    for (int i = 0; i < n; ++i) slots += args[i].chunkSlots;
+    for (int i = 0; i < n; ++i) {
+        hArgs[i] = args[i];
+        if (slots > 2048) hArgs[i].segs = 1;
+    }
     int k = n;
     for (int cl = 0; cl < 2; ++cl)
-        for (int i : occIdx[cl]) hArgs[k++] = args[i];
+        for (int i : occIdx[cl]) hArgs[k++] = hArgs[i];
     int* hMaps = (int*)(image.data() + argBytes);
     const FdArgs* dArgs = (const FdArgs*)devBase;
     const int* dMaps = (const int*)(devBase + argBytes);
@@ -596,7 +603,7 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
         int acc = 0;
         mp[0] = 0;
         for (int q = 0; q < cnt; ++q) {
-            const FdArgs& a = args[occ ? (*order)[q] : q];
+            const FdArgs& a = hArgs[occ ? (*order)[q] : q];
             const int nb = 1 << (2 * a.gridLog2);
             int blocks = 1;
             switch (m) {

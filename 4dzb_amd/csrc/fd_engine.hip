@@ -184,6 +184,8 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     a.triCount = (int*)take(T * sizeof(int));
     a.triRange = (uint2*)take(T * sizeof(uint2));
     a.histT = (uint32_t*)take(a.nT * sizeof(uint32_t));
+    a.triOff = (uint32_t*)take(T * sizeof(uint32_t));
+    a.pairKT = (uint32_t*)take((size_t)a.pairCap * sizeof(uint32_t));
     a.sortedTri = (int*)take((size_t)a.pairCap * sizeof(int));
     int* tb = (int*)take((size_t)(5 * (nb + 2)) * sizeof(int));
     a.rxStart = tb;
@@ -191,7 +193,8 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     a.triStart = tb ? tb + 2 * (nb + 2) : nullptr;
     a.triEnd = tb ? tb + 3 * (nb + 2) : nullptr;
     a.chunkStart = tb ? tb + 4 * (nb + 2) : nullptr;
-    const long np = std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)) + 1;
+    const long np = std::max(std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)),
+                             fdk::scan_partials_needed(T)) + 1;
     a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));
     a.opCount = countOps ? (unsigned long long*)take(FD_NOPS * sizeof(unsigned long long)) : nullptr;
     if (!a.masks) a.masks = (uint32_t*)take(WH * a.L.res * a.L.wpr * sizeof(uint32_t));
@@ -614,6 +617,7 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
                 case FD_MAP_SCANR: blocks = fdk::scan_chunks(a.nR); break;
                 case FD_MAP_TRI: blocks = a.nblkT; break;
                 case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
+                case FD_MAP_SCANO: blocks = fdk::scan_chunks(std::max(a.ntris, 1)); break;
                 case FD_MAP_ZERO: blocks = a.segs > 1 ? fdk::zero_blocks(a) : 0; break;
                 case FD_MAP_FIN: blocks = a.segs > 1 ? a.chunkSlots : 0; break;
                 default: blocks = a.chunkSlots * a.segs; break;

@@ -45,6 +45,8 @@ struct FdArgs {
     int* triCount;        // ntris
     uint2* triRange;      // ntris: (sx | sy << 16, ex | ey << 16)
     uint32_t* histT;      // nb * nblkT, scanned in place
+    uint32_t* triOff;     // ntris: pair counts, scanned in place into triangle-major offsets
+    uint32_t* pairKT;     // pairCap: triangle-major pairs, (index in block << 16) | bin
     int* sortedTri;       // pairCap
     unsigned long long pairCap;
     int* rxStart;         // nb + 1
@@ -70,7 +72,7 @@ struct FdArgs {
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
 enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,
-             FD_MAP_OCC64, FD_MAP_ZERO, FD_MAP_FIN, FD_NMAPS };
+             FD_MAP_OCC64, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO, FD_NMAPS };
 
 struct FdBatchPlan {
     const FdArgs* inst;       // device: instance table (n)

@@ -619,19 +619,24 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     if (!act) { c.ok = false; c.sx = c.sy = c.ex_ = c.ey_ = 0; }
     int count = 0, tsx = 6000, tsy = 6000, tex = 0, tey = 0;
     const bool has = c.ok && c.sx < c.ex_ && c.sy < c.ey_;
-    for_covered_bins(has, c.sx, c.sy, c.ex_, c.ey_, log2G, [&](int bp, int i, int j, bool mine) {
-        const BinStat b = load_bin(a.binStat + 5 * bp);
-        const bool pass = mine && bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
-        const uint64_t bal = __ballot(pass);
-        if (pass) {
-            ++count;
-            tsx = min(tsx, i); tsy = min(tsy, j);
-            tex = max(tex, i + 1); tey = max(tey, j + 1);
-        }
-        if (bal && lane_id() == 0) atomicAdd(&thist[bp], (uint32_t)__popcll(bal));
-    });
+    // every lane walks its own bin rectangle (bin statistics by vector loads
+    // from L2); per-block bin counts by LDS atomics
+    if (has) {
+        for (int j = c.sy; j < c.ey_; ++j)
+            for (int i = c.sx; i < c.ex_; ++i) {
+                const int bp = j << log2G | i;
+                const BinStat b = load_bin(a.binStat + 5 * bp);
+                if (bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey)) {
+                    ++count;
+                    tsx = min(tsx, i); tsy = min(tsy, j);
+                    tex = max(tex, i + 1); tey = max(tey, j + 1);
+                    atomicAdd(&thist[bp], 1u);
+                }
+            }
+    }
     if (act) {
         a.triCount[t] = count;
+        a.triOff[t] = (uint32_t)count;
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
     }
     __syncthreads();
@@ -640,38 +645,74 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
 
 // Stable pair scatter (writeTriBinPair_kernel + sort_by_key,
 // Step4_RasterizeTriangleToBin.inl:161-223): per bin, triangles ascending.
-__device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk) {
-    extern __shared__ int tcnt[];  // nb, -1 = not loaded yet
+// Pair write: every triangle writes its (bin) pairs at its triangle-major
+// offset (scanned pair counts), bins ascending within the triangle -- the
+// order in which the reference's writeTriBinPair_kernel emits them
+// (Step4_RasterizeTriangleToBin.inl:161-204), with the count pass's test.
+__device__ __forceinline__ void tri_write_body(const FdArgs& a, const int blk) {
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
-    const int nb = s->nb, log2G = s->log2G;
-    for (int k = threadIdx.x; k < nb; k += 64) tcnt[k] = -1;
-    __syncthreads();
+    const int log2G = s->log2G;
+    const int t = blk * FD_TRB + threadIdx.x;
+    if (t >= a.ntris) return;
+    const uint2 rg = a.triRange[t];
+    const int sx = rg.x & 0xffff, sy = rg.x >> 16, ex = rg.y & 0xffff, ey = rg.y >> 16;
+    if (sx >= ex || sy >= ey) return;
+    const TriClamp c = tri_clamp(a.triAABB[2 * t], a.triAABB[2 * t + 1], s);
+    if (!c.ok) return;
+    unsigned long long o = a.triOff[t];
     const unsigned long long cap = a.pairCap;
-    for (int it = 0; it < FD_TRB / 64; ++it) {
-        const int t = blk * FD_TRB + it * 64 + lane_id();
-        const bool act = t < a.ntris;
-        float4 mn = make_float4(0, 0, 0, 0), mx = mn;
-        uint2 rg = make_uint2(6000u | (6000u << 16), 0u);
-        if (act) { mn = a.triAABB[2 * t]; mx = a.triAABB[2 * t + 1]; rg = a.triRange[t]; }
-        const TriClamp c = tri_clamp(mn, mx, s);
-        const int sx = rg.x & 0xffff, sy = rg.x >> 16, ex = rg.y & 0xffff, ey = rg.y >> 16;
-        const bool has = act && c.ok && sx < ex && sy < ey;
-        for_covered_bins(has, sx, sy, ex, ey, log2G, [&](int bp, int, int, bool mine) {
-            const BinStat b = load_bin(a.binStat + 5 * bp);
-            const bool pass = mine && bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey);
-            const uint64_t bal = __ballot(pass);
-            if (bal) {
-                int b0 = tcnt[bp];
-                if (b0 < 0) b0 = (int)a.histT[(size_t)bp * a.nblkT + blk];
-                if (pass) {
-                    const unsigned long long p = (unsigned long long)(uint32_t)b0 + (unsigned)mbcnt(bal);
-                    if (p < cap) a.sortedTri[p] = t;
-                }
-                if (lane_id() == 0) tcnt[bp] = b0 + __popcll(bal);
+    const uint32_t tag = (uint32_t)threadIdx.x << 16;
+    for (int j = sy; j < ey; ++j)
+        for (int i = sx; i < ex; ++i) {
+            const int bp = j << log2G | i;
+            if (bin_test(load_bin(a.binStat + 5 * bp), c.bx, c.by, c.bz, c.ex, c.ey)) {
+                if (o < cap) a.pairKT[o] = tag | (uint32_t)bp;
+                ++o;
             }
-        });
-        __syncthreads();
+        }
+}
+
+// Stable counting scatter of one triangle block's pairs (the reference's
+// stable sort_by_key by bin): one wave walks the block's triangle-major pair
+// list in chunks of 64, lanes with equal bins are matched by ballots over the
+// key bits and ranked with mbcnt, and per-bin running offsets live in LDS
+// (first use: the block's scanned per-bin offset).  Result: bin-major,
+// ascending triangle id within a bin.
+__device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk) {
+    extern __shared__ int cnt[];  // nb, -1 = not loaded yet
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb;
+    int kbits = 1;
+    while ((1 << kbits) < nb) ++kbits;
+    for (int k = threadIdx.x; k < nb; k += 64) cnt[k] = -1;
+    __syncthreads();
+    const int t0 = blk * FD_TRB;
+    const int t1 = min(t0 + FD_TRB, a.ntris);
+    const unsigned long long p0 = a.triOff[t0];
+    const unsigned long long p1 = t1 < a.ntris ? (unsigned long long)a.triOff[t1] : s->pairs;
+    const unsigned long long cap = a.pairCap;
+    for (unsigned long long c0 = p0; c0 < p1; c0 += 64) {
+        const unsigned long long p = c0 + lane_id();
+        const bool act = p < p1 && p < cap;
+        const uint32_t v = act ? a.pairKT[p] : 0u;
+        const int key = (int)(v & 0xffffu);
+        uint64_t eq = __ballot(act);
+        for (int b = 0; b < kbits; ++b) {
+            const bool bit = (key >> b) & 1;
+            const uint64_t bal = __ballot(act && bit);
+            eq &= bit ? bal : ~bal;
+        }
+        if (act) {
+            const int rank = mbcnt(eq);
+            const int gcount = __popcll(eq);
+            int b0 = cnt[key];
+            if (b0 < 0) b0 = (int)a.histT[(size_t)key * a.nblkT + blk];
+            const unsigned long long dst = (unsigned long long)(uint32_t)b0 + (unsigned)rank;
+            if (rank == gcount - 1) cnt[key] = b0 + gcount;
+            if (dst < cap) a.sortedTri[dst] = t0 + (int)(v >> 16);
+        }
     }
 }
 
@@ -1372,6 +1413,7 @@ __global__ void k_finalize(FdArgs a) { finalize_body(a, blockIdx.x); }
 __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) { rx_key_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) { rx_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) { tri_count_body(a, blockIdx.x); }
+__global__ __launch_bounds__(FD_TRB) void k_tri_write(FdArgs a) { tri_write_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) { tri_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
 __global__ void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
@@ -1434,6 +1476,7 @@ FD_BATCHED(kb_finalize, 64, finalize_body)
 FD_BATCHED(kb_rx_key, 256, rx_key_body)
 FD_BATCHED(kb_rx_scatter, 64, rx_scatter_body)
 FD_BATCHED(kb_tri_count, FD_TRB, tri_count_body)
+FD_BATCHED(kb_tri_write, FD_TRB, tri_write_body)
 FD_BATCHED(kb_tri_scatter, 64, tri_scatter_body)
 FD_BATCHED(kb_tables, 1024, tables_body)
 FD_BATCHED(kb_post_pairs, 64, post_pairs_body)
@@ -1460,16 +1503,24 @@ __global__ __launch_bounds__(64) void kb_occlude_finish(const FdArgs* __restrict
 }
 
 // which = 0: receiver key histogram, 1: pair histogram
+// The three scanned arrays of a frame: 0 receiver histogram, 1 pair
+// histogram, 2 per-triangle pair counts (the last two have the same total).
+__device__ __forceinline__ uint32_t* scan_data(const FdArgs& a, int which) {
+    return which == 0 ? a.histR : which == 1 ? a.histT : a.triOff;
+}
+__device__ __forceinline__ long scan_len(const FdArgs& a, int which) {
+    return which == 0 ? a.nR : which == 1 ? a.nT : (long)a.ntris;
+}
 __global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_reduce(const FdArgs* __restrict__ inst, const int* __restrict__ map,
                                                              int n, int which) {
     const int b = blockIdx.x;
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
-    scan_reduce_body(which ? a.histT : a.histR, which ? a.nT : a.nR, a.scanPart, &a.st->status, b - map[i]);
+    scan_reduce_body(scan_data(a, which), scan_len(a, which), a.scanPart, &a.st->status, b - map[i]);
 }
 __global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_partials(const FdArgs* __restrict__ inst, int which) {
     const FdArgs a = inst[blockIdx.x];
-    const long len = which ? a.nT : a.nR;
+    const long len = scan_len(a, which);
     scan_partials_body(a.scanPart, (int)((len + SCAN_CHUNK - 1) / SCAN_CHUNK), which ? &a.st->pairs : &a.st->rxTotal,
                        &a.st->status);
 }
@@ -1478,7 +1529,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_apply(const FdArgs* __rest
     const int b = blockIdx.x;
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
-    scan_apply_body(which ? a.histT : a.histR, which ? a.nT : a.nR, a.scanPart, &a.st->status, b - map[i]);
+    scan_apply_body(scan_data(a, which), scan_len(a, which), a.scanPart, &a.st->status, b - map[i]);
 }
 
 // Verification of the row-mask primitive on given (D, B) pairs (n = R - 1),
@@ -1567,7 +1618,13 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         kb_scan_apply<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
     }
     kb_post_pairs<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
-    if (p.anyTris) kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+    if (p.anyTris) {
+        kb_scan_reduce<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
+        kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 2);
+        kb_scan_apply<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
+        kb_tri_write<<<L(FD_MAP_TRI), FD_TRB, 0, st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+    }
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
     for (int c = 0; c < 2; ++c) {
@@ -1634,6 +1691,9 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         e = scan_u32(a.histT, a.nT, a.scanPart, &s->pairs, &s->status, st);
         if (e != hipSuccess) return e;
         k_post_pairs<<<1, 64, 0, st>>>(a);
+        e = scan_u32(a.triOff, a.ntris, a.scanPart, &s->pairs, &s->status, st);
+        if (e != hipSuccess) return e;
+        k_tri_write<<<a.nblkT, FD_TRB, 0, st>>>(a);
         k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);
     } else {
         k_post_pairs<<<1, 64, 0, st>>>(a);

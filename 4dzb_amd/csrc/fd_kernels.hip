@@ -981,22 +981,27 @@ __device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (
 // t0 and E/|al| come from v_rcp_f32 (2^-22 relative); a margin of 2^-10
 // covers that while |t0| <= 2^11, and a pair is used only when
 // E/|al| <= 2^10, so that beyond |t0| = 2^11 the window is far outside
-// [0, R-1] either way.  Pairs with |al| < 2^-40, and lanes with any |A|, |B|,
-// |C| above 2^40 (where the products could overflow), are not used; NaN
-// thresholds drop out of fmaxf / fminf.
+// [0, R-1] either way.  The bounds t0 -/+ (E/|al| + 2^-10) are formed with
+// one fma each (t0 - m = -(be * ra + m), t0 + m = -be * ra + m), which only
+// rounds less than a product followed by a sum.  Pairs with |al| < 2^-40,
+// and lanes with any |A|, |B|, |C| above 2^40 (where the products could
+// overflow), are not used; every bound that is used is then finite.
 template <typename T>
 __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB) {
     const float u20 = 9.5367431640625e-07f;   // 2^-20
     const float tiny = 1.17549435e-38f;       // 2^-126
     const float lim = 1.09951163e+12f;        // 2^40
-    // mAB = max |A|, |B| (NaN-free maximum); a NaN anywhere only makes the
-    // thresholds NaN, which fmaxf / fminf below drop
+    // mAB = max |A|, |B| (NaN-free maximum); a NaN coefficient makes the
+    // lane's pair terms NaN, which fail the |al| test and are not used
     const float mC = fmaxf(fmaxf(fabsf(e[0].C), fabsf(e[1].C)), fabsf(e[2].C));
     const bool ok = fmaxf(mAB, mC) <= lim;
     const float tau = __builtin_fmaf(__builtin_fmaf(mAB, 2.0f * Rf, mC), u20, tiny);
     const float Cp[3] = {e[0].C + tau, e[1].C + tau, e[2].C + tau};
     const float E = __builtin_fmaf(__builtin_fmaf(mAB, Rf, mC + tau), mAB * 1.9073486328125e-06f, tiny);
-    float lo = 0.0f, hi = Rf - 1.0f;
+    bool pos[3], neg[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { pos[k] = e[k].B > 0.0f; neg[k] = e[k].B < 0.0f; }
+    float lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int i = k, j = (k + 1) % 3;
@@ -1004,16 +1009,17 @@ __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB)
         const float pB = e[i].B * Cp[j], qB = e[j].B * Cp[i];
         const float al = pA - qA, be = pB - qB;
         const float ra = __builtin_amdgcn_rcpf(al);
-        const float t0 = -be * ra;
         const float m = __builtin_fmaf(E, fabsf(ra), 9.765625e-4f);   // E/|al| + 2^-10
-        const bool opp = (e[i].B > 0.0f && e[j].B < 0.0f) || (e[i].B < 0.0f && e[j].B > 0.0f);
+        const bool opp = (pos[i] && neg[j]) || (neg[i] && pos[j]);
         const bool use = ok && opp && fabsf(al) >= 9.09494702e-13f && m <= 1024.0f;
-        const bool lower = (e[i].B > 0.0f) == (al > 0.0f);
-        lo = fmaxf(lo, (use && lower) ? t0 - m : 0.0f);
-        hi = fminf(hi, (use && !lower) ? t0 + m : Rf - 1.0f);
+        const bool lower = pos[i] == (al > 0.0f);
+        lo[k] = (use && lower) ? -__builtin_fmaf(be, ra, m) : 0.0f;
+        hi[k] = (use && !lower) ? __builtin_fmaf(-be, ra, m) : Rf - 1.0f;
     }
-    const float lc = ceilf(fminf(lo, Rf));
-    const float hc = floorf(fmaxf(hi, -1.0f));
+    const float l = __builtin_amdgcn_fmed3f(fmaxf(fmaxf(lo[0], lo[1]), lo[2]), 0.0f, Rf);
+    const float h = __builtin_amdgcn_fmed3f(fminf(fminf(hi[0], hi[1]), hi[2]), -1.0f, Rf - 1.0f);
+    const float lc = ceilf(l);
+    const float hc = floorf(h);
     const unsigned w = (unsigned)fmaxf(hc + 1.0f - lc, 0.0f);
     return interval_bits<T>(w, (unsigned)lc);
 }
@@ -1181,18 +1187,18 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 #endif
     const cint_p triList = (cint_p)(a.sortedTri);
     const ctri_p tris = (ctri_p)(a.tri);
+    bool live = act;   // receiver in this wave and not yet fully shadowed
     for (int ti = ts; ti < te; ++ti) {
-        const bool live = act && full != FULL;
         const uint64_t liveMask = __ballot(live);
         if (liveMask == 0) break;
         if (COUNT) { nCand += __popcll(liveMask); ++nWaveIt; }
         const int tid = triList[ti];
         if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
-        if (!live) continue;
         // containment + depth cull (:120-127)
-        if (!(t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx && q.rmaxy <= t.bmaxy && t.minz <= q.z))
-            continue;
+        const bool pass = live && t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx &&
+                          q.rmaxy <= t.bmaxy && t.minz <= q.z;
+        if (!pass) continue;
         if (COUNT) ++nPass;
         // dir = dot(Q - v0, n) (:130-132)
         const float dx = q.q[0] - t.v0[0], dy = q.q[1] - t.v0[1], dz = q.q[2] - t.v0[2];
@@ -1237,14 +1243,15 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 e[k].B = dot3(nv[k][0], nv[k][1], nv[k][2], yd0, yd1, yd2);
             }
         }
+        // |A|, |B| <= 2^100 for the fast mask form (a NaN dropped by fmaxf is
+        // harmless there: it makes both f1 and f2 false, as in the reference);
+        // taken before the negation below, which does not change magnitudes
+        const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fabsf(e[2].A)),
+                                fmaxf(fmaxf(fabsf(e[0].B), fabsf(e[1].B)), fabsf(e[2].B)));
         if (dir < 0) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) { e[k].A = -e[k].A; e[k].B = -e[k].B; e[k].C = -e[k].C; }
         }
-        // |A|, |B| <= 2^100 for the fast mask form (a NaN dropped by fmaxf is
-        // harmless there: it makes both f1 and f2 false, as in the reference)
-        const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fmaxf(fabsf(e[2].A), fabsf(e[0].B))),
-                                fmaxf(fabsf(e[1].B), fabsf(e[2].B)));
         const bool big = !(mAB <= 1.2676506e+30f);
         if (COUNT) {
             // the rows the reference visits for this pair: its row-range mask
@@ -1263,42 +1270,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             }
             nRefRows += Bits<T>::popc((~full) & FULL & three_edges<T, true>(g, DNg, Rf, R));
         }
-        // the fast mask form needs finite limits in every lane of the wave
-        const bool fast = !__any(big);
-        T win = FULL;
-        if (fast) {
-            win = row_window<T>(e, Rf, mAB);
-            if (!__any(win != (T)0)) continue;   // no lane can gain a bit from this triangle
-        }
-        if (!COUNT && onTri && win != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            e[k].nRowN = -(e[k].A * nsub);
-            e[k].nColN = -(e[k].B * nsub);
-            e[k].rA = rcp_or_nan(e[k].A);
-            e[k].rB = rcp_or_nan(e[k].B);
-        }
-        // rows that may hold occluded samples (:176-191)
-        float DN[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) DN[k] = (e[k].B > 0 ? -e[k].nColN : 0.0f) + e[k].C;
-        T valid;
-        if (fast) {
-            const float nR[3] = {e[0].nRowN, e[1].nRowN, e[2].nRowN};
-            const float rA[3] = {e[0].rA, e[1].rA, e[2].rA};
-            const float dA[3] = {e[0].A, e[1].A, e[2].A};
-            valid = (~full) & win & three_edges_fast<T>(DN, nR, rA, dA, Rf);
-        } else {
-            valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
-        }
-        if (COUNT) {
-            // wave row iterations = max over lanes of this triangle's rows
-            int nr = Bits<T>::popc(valid);
-            for (int o = 32; o >= 1; o >>= 1) nr = max(nr, __shfl_xor(nr, o, 64));
-            if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) nWaveRows += nr;
-        }
         // row by row, lowest row first (:204-222)
-        auto rows = [&](auto rowMask) {
+        auto rows = [&](T valid, auto rowMask) {
             while (valid) {
                 if (COUNT) ++nRows;
                 const int r = Bits<T>::ctz(valid);
@@ -1324,14 +1297,54 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 valid &= valid - (T)1;
             }
         };
-        if (fast) {
-            const float nC[3] = {e[0].nColN, e[1].nColN, e[2].nColN};
-            const float rB[3] = {e[0].rB, e[1].rB, e[2].rB};
-            const float dB[3] = {e[0].B, e[1].B, e[2].B};
-            rows([&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rf); });
+        auto wave_rows = [&](T valid) {
+            if (COUNT) {
+                // wave row iterations = max over lanes of this triangle's rows
+                int nr = Bits<T>::popc(valid);
+                for (int o = 32; o >= 1; o >>= 1) nr = max(nr, __shfl_xor(nr, o, 64));
+                if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) nWaveRows += nr;
+            }
+        };
+        if (!__any(big)) {
+            // fast form: finite limits in every lane of the wave
+            const T win = row_window<T>(e, Rf, mAB);
+            if (!__any(win != (T)0)) continue;   // no lane can gain a bit from this triangle
+            if (!COUNT && onTri && win != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+            // |A|, |B| <= 2^100 here, so only the lower end of the reciprocal
+            // range is checked (NaN coefficients fail it as well);
+            // DN = C + max(B (R-1), 0) equals (B > 0 ? B (R-1) : 0) + C up to
+            // the sign of a zero, B being finite or NaN
+            float nR[3], rA[3], dA[3], nC[3], rB[3], dB[3], DN[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                dA[k] = e[k].A;
+                dB[k] = e[k].B;
+                nR[k] = -(e[k].A * nsub);
+                nC[k] = -(e[k].B * nsub);
+                rA[k] = fabsf(e[k].A) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].A) : __builtin_nanf("");
+                rB[k] = fabsf(e[k].B) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].B) : __builtin_nanf("");
+                DN[k] = e[k].C + fmaxf(-nC[k], 0.0f);
+            }
+            // rows that may hold occluded samples (:176-191)
+            const T valid = (~full) & win & three_edges_fast<T>(DN, nR, rA, dA, Rf);
+            wave_rows(valid);
+            rows(valid, [&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rf); });
         } else {
-            rows([&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
+            if (!COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+            float DN[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                e[k].nRowN = -(e[k].A * nsub);
+                e[k].nColN = -(e[k].B * nsub);
+                e[k].rA = rcp_or_nan(e[k].A);
+                e[k].rB = rcp_or_nan(e[k].B);
+                DN[k] = (e[k].B > 0 ? -e[k].nColN : 0.0f) + e[k].C;
+            }
+            const T valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
+            wave_rows(valid);
+            rows(valid, [&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
         }
+        if (full == FULL) live = false;
     }
     // fraction = popcount / (W*H) (:226-231); split lists: after the merge
     if (S == 1) {

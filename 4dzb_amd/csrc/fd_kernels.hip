@@ -921,15 +921,19 @@ __device__ __forceinline__ int min3i(int a, int b, int c) {
 //    fl(B*n + D) >= 0  <=>  D >= -fl(B*n)  -- also for infinite D.
 //  * kk: only read when f1 != f2.  Then the exact quotient q lies in
 //    [0, n + ulp] (f1 && !f2: D >= 0 > B; !f1 && f2: D < 0 < B, both with
-//    |D| <= |B*n|).  x = fma(D, rcp(B), R) is within |q| 2^-22 + 2^-19 <
-//    2^-12 of R - q; when x is more than 2^-12 away from every integer,
-//    floor(x) = R - ceil(q) = R - kk exactly, in [1, R-1] (no clamp).
-//    The lanes left (x near an integer, NaN reciprocal) divide exactly.
+//    |D| <= |B*n|).  X = D * rcp(B) + R is within |q| 2^-22 + 2^-19 < 2^-15
+//    of Y = R - q.  The kernel forms x = fma(D, rcp(B), R + 2^-12) (Rs is
+//    R + 2^-12, exact), which rounds X + 2^-12 by at most 2^-18.  When
+//    fract(x) >= 2^-11, with k = floor(x):  X lies in
+//    [k + 2^-12 - 2^-18, k + 1 - 2^-12 + 2^-18), so Y lies strictly inside
+//    (k, k + 1) and kk = ceil(q) = R - floor(Y) = R - k, in [1, R-1] (no
+//    clamp).  The lanes left (fract below 2^-11, or NaN from a NaN
+//    reciprocal) divide exactly.
 //  * with c = R - kk the AND is [R - min over suffix edges of c,
 //    R - max over prefix edges of c), "none" edges giving the empty set.
 template <typename T>
 __device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (&nN)[3], const float (&r)[3],
-                                              const float (&div)[3], float Rf) {
+                                              const float (&div)[3], float Rs, int R) {
     bool f1[3], f2[3], slow[3];
     int c[3];
     bool anySlow = false;
@@ -937,18 +941,16 @@ __device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (
     for (int k = 0; k < 3; ++k) {
         f1[k] = D[k] >= 0.0f;
         f2[k] = D[k] >= nN[k];
-        const float x = __builtin_fmaf(D[k], r[k], Rf);
+        const float x = __builtin_fmaf(D[k], r[k], Rs);
         asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(c[k]) : "v"(x));   // floor(x), saturating, NaN -> 0
-        slow[k] = !(fabsf(x - rintf(x)) > 2.44140625e-4f) && (f1[k] != f2[k]);
+        slow[k] = !(__builtin_amdgcn_fractf(x) >= 4.8828125e-4f) && (f1[k] != f2[k]);
         anySlow |= slow[k];
     }
     if (anySlow) {
-        const int R = (int)Rf;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
             if (slow[k]) c[k] = R - k_exact(D[k], div[k], R);
     }
-    const int R = (int)Rf;
     const int hiC = max3i((f1[0] && !f2[0]) ? c[0] : 0, (f1[1] && !f2[1]) ? c[1] : 0, (f1[2] && !f2[2]) ? c[2] : 0);
     const int loC = min3i((!f1[0] && f2[0]) ? c[0] : R, (!f1[1] && f2[1]) ? c[1] : R, (!f1[2] && f2[2]) ? c[2] : R);
     const bool none = (!f1[0] && !f2[0]) || (!f1[1] && !f2[1]) || (!f1[2] && !f2[2]);
@@ -983,9 +985,12 @@ __device__ __forceinline__ T three_edges_fast(const float (&D)[3], const float (
 // E/|al| <= 2^10, so that beyond |t0| = 2^11 the window is far outside
 // [0, R-1] either way.  The bounds t0 -/+ (E/|al| + 2^-10) are formed with
 // one fma each (t0 - m = -(be * ra + m), t0 + m = -be * ra + m), which only
-// rounds less than a product followed by a sum.  Pairs with |al| < 2^-40,
-// and lanes with any |A|, |B|, |C| above 2^40 (where the products could
-// overflow), are not used; every bound that is used is then finite.
+// rounds less than a product followed by a sum.  E is raised to at least
+// 2^-30 (a larger E only widens the window), so m <= 2^10 also keeps
+// |ra| <= 2^40, i.e. |al| >= 2^-40: al is a normal number, and a zero or
+// denormal al (infinite or huge reciprocal) is never used.  Lanes with any
+// |A|, |B|, |C| above 2^40 (where the products could overflow) are not
+// used; every bound that is used is then finite.
 template <typename T>
 __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB) {
     const float u20 = 9.5367431640625e-07f;   // 2^-20
@@ -997,7 +1002,8 @@ __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB)
     const bool ok = fmaxf(mAB, mC) <= lim;
     const float tau = __builtin_fmaf(__builtin_fmaf(mAB, 2.0f * Rf, mC), u20, tiny);
     const float Cp[3] = {e[0].C + tau, e[1].C + tau, e[2].C + tau};
-    const float E = __builtin_fmaf(__builtin_fmaf(mAB, Rf, mC + tau), mAB * 1.9073486328125e-06f, tiny);
+    const float E = fmaxf(__builtin_fmaf(__builtin_fmaf(mAB, Rf, mC + tau), mAB * 1.9073486328125e-06f, tiny),
+                          9.31322575e-10f);   // >= 2^-30
     bool pos[3], neg[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { pos[k] = e[k].B > 0.0f; neg[k] = e[k].B < 0.0f; }
@@ -1011,17 +1017,18 @@ __device__ __forceinline__ T row_window(const Edge (&e)[3], float Rf, float mAB)
         const float ra = __builtin_amdgcn_rcpf(al);
         const float m = __builtin_fmaf(E, fabsf(ra), 9.765625e-4f);   // E/|al| + 2^-10
         const bool opp = (pos[i] && neg[j]) || (neg[i] && pos[j]);
-        const bool use = ok && opp && fabsf(al) >= 9.09494702e-13f && m <= 1024.0f;
+        const bool use = ok && opp && m <= 1024.0f;
         const bool lower = pos[i] == (al > 0.0f);
         lo[k] = (use && lower) ? -__builtin_fmaf(be, ra, m) : 0.0f;
         hi[k] = (use && !lower) ? __builtin_fmaf(-be, ra, m) : Rf - 1.0f;
     }
     const float l = __builtin_amdgcn_fmed3f(fmaxf(fmaxf(lo[0], lo[1]), lo[2]), 0.0f, Rf);
     const float h = __builtin_amdgcn_fmed3f(fminf(fminf(hi[0], hi[1]), hi[2]), -1.0f, Rf - 1.0f);
-    const float lc = ceilf(l);
-    const float hc = floorf(h);
-    const unsigned w = (unsigned)fmaxf(hc + 1.0f - lc, 0.0f);
-    return interval_bits<T>(w, (unsigned)lc);
+    int nlc, hc;   // -ceil(l) = floor(-l), floor(h)
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(nlc) : "v"(-l));
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hc) : "v"(h));
+    const unsigned w = (unsigned)max(hc + 1 + nlc, 0);
+    return interval_bits<T>(w, (unsigned)(-nlc));
 }
 
 // test3DPointInTriangle, Step5_2_shadowCal.inl:213-222
@@ -1173,6 +1180,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const bool axial = xd1 == 0.0f && xd2 == 0.0f && yd0 == 0.0f && yd2 == 0.0f;
     const float nsub = (float)(unsigned)(R - 1);
     const float Rf = (float)R;
+    const float Rs = Rf + 2.44140625e-4f;   // R + 2^-12, exact
     const float eps = a.eps;
     T full = 0;
     unsigned long long nCand = 0, nPass = 0, nRows = 0, nWaveIt = 0, nWaveRows = 0, nRefRows = 0, nPlanes = 0;
@@ -1200,58 +1208,68 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                           q.rmaxy <= t.bmaxy && t.minz <= q.z;
         if (!pass) continue;
         if (COUNT) ++nPass;
-        // dir = dot(Q - v0, n) (:130-132)
-        const float dx = q.q[0] - t.v0[0], dy = q.q[1] - t.v0[1], dz = q.q[2] - t.v0[2];
-        const float dir = dot3(dx, dy, dz, t.n[0], t.n[1], t.n[2]);
+        // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172) start from
+        // a_k = v_{k+1} - Q; a_2 = v0 - Q also gives
+        // dir = dot(Q - v0, n) (:130-132) = -dot(v0 - Q, n), exactly (every
+        // rounding commutes with the negation)
+        const float* vb[3] = {t.v1, t.v2, t.v0};
+        const float* ev[3] = {t.e01, t.e12, t.e20};
+        float av[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            av[k][0] = vb[k][0] - q.q[0];
+            av[k][1] = vb[k][1] - q.q[1];
+            av[k][2] = vb[k][2] - q.q[2];
+        }
+        const float ndir = dot3(av[2][0], av[2][1], av[2][2], t.n[0], t.n[1], t.n[2]);   // -dir
         // Q on the triangle: the reference skips the pair (:134-135).  The
         // counting variant tests here, in the reference's order; the timed
         // variant tests below, only for pairs that can still set a bit (the
         // outcome is the same: a skipped pair sets nothing).
-        const bool onTri = fabsf(dir) < eps;
+        const bool onTri = fabsf(ndir) < eps;
         if (COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
-        // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172).  The reference
-        // negates t0, t1, L0_Q when dir < 0; negation commutes with every
-        // rounding here, so A, B, C are negated instead (identical values up
-        // to the sign of zero results, which no later step can observe).
+        // The reference negates t0, t1, L0_Q when dir < 0.  The sign is put
+        // into L0 - Q and the light axes before the products (a sign flip
+        // commutes with every rounding): A, B, C come out negated, identical
+        // up to the sign of zero results, which no later step can observe.
+        const uint32_t sg = (ndir > 0.0f) ? 0x80000000u : 0u;   // dir < 0
+        auto flip = [sg](float v) { return __uint_as_float(__float_as_uint(v) ^ sg); };
+        const float Lxs = flip(Lx), Lys = flip(Ly), Lzs = flip(Lz);
         Edge e[3];
-        const float* vb[3] = {t.v1, t.v2, t.v0};
-        const float* ev[3] = {t.e01, t.e12, t.e20};
         float nv[3][3];
         bool finiteC = true;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float ax = vb[k][0] - q.q[0], ay = vb[k][1] - q.q[1], az = vb[k][2] - q.q[2];
+            const float ax = av[k][0], ay = av[k][1], az = av[k][2];
             const float bx = ev[k][0], by = ev[k][1], bz = ev[k][2];
             nv[k][0] = ay * bz - az * by;
             nv[k][1] = az * bx - ax * bz;
             nv[k][2] = ax * by - ay * bx;
-            e[k].C = dot3(nv[k][0], nv[k][1], nv[k][2], Lx, Ly, Lz);
+            e[k].C = dot3(nv[k][0], nv[k][1], nv[k][2], Lxs, Lys, Lzs);
             finiteC = finiteC && __builtin_isfinite(e[k].C);
         }
         // a finite C = dot(n, L0 - Q) implies a finite n (an infinite
         // component makes a product infinite or NaN)
         if (axial && !__any(!finiteC)) {
+            const float xs = flip(xd0), ys = flip(yd1);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                e[k].A = nv[k][0] * xd0;
-                e[k].B = nv[k][1] * yd1;
+                e[k].A = nv[k][0] * xs;
+                e[k].B = nv[k][1] * ys;
             }
         } else {
+            const float x0 = flip(xd0), x1 = flip(xd1), x2 = flip(xd2);
+            const float y0 = flip(yd0), y1 = flip(yd1), y2 = flip(yd2);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                e[k].A = dot3(nv[k][0], nv[k][1], nv[k][2], xd0, xd1, xd2);
-                e[k].B = dot3(nv[k][0], nv[k][1], nv[k][2], yd0, yd1, yd2);
+                e[k].A = dot3(nv[k][0], nv[k][1], nv[k][2], x0, x1, x2);
+                e[k].B = dot3(nv[k][0], nv[k][1], nv[k][2], y0, y1, y2);
             }
         }
         // |A|, |B| <= 2^100 for the fast mask form (a NaN dropped by fmaxf is
-        // harmless there: it makes both f1 and f2 false, as in the reference);
-        // taken before the negation below, which does not change magnitudes
+        // harmless there: it makes both f1 and f2 false, as in the reference)
         const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fabsf(e[2].A)),
                                 fmaxf(fmaxf(fabsf(e[0].B), fabsf(e[1].B)), fabsf(e[2].B)));
-        if (dir < 0) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) { e[k].A = -e[k].A; e[k].B = -e[k].B; e[k].C = -e[k].C; }
-        }
         const bool big = !(mAB <= 1.2676506e+30f);
         if (COUNT) {
             // the rows the reference visits for this pair: its row-range mask
@@ -1323,12 +1341,13 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 nC[k] = -(e[k].B * nsub);
                 rA[k] = fabsf(e[k].A) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].A) : __builtin_nanf("");
                 rB[k] = fabsf(e[k].B) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].B) : __builtin_nanf("");
+                asm("" : "+v"(rB[k]));   // keep the select out of the row loop
                 DN[k] = e[k].C + fmaxf(-nC[k], 0.0f);
             }
             // rows that may hold occluded samples (:176-191)
-            const T valid = (~full) & win & three_edges_fast<T>(DN, nR, rA, dA, Rf);
+            const T valid = (~full) & win & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
             wave_rows(valid);
-            rows(valid, [&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rf); });
+            rows(valid, [&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rs, R); });
         } else {
             if (!COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
             float DN[3];
@@ -1564,12 +1583,13 @@ __global__ void k_selftest_rowmask(const float* D, const float* B, int n, int R,
         nC[k] = e[k].nColN; rB[k] = e[k].rB; dB[k] = B[i];
     }
     const bool fast = mode == 1 && fabsf(B[i]) <= 1.2676506e+30f;
+    const float Rs = (float)R + 2.44140625e-4f;   // R + 2^-12
     const uint64_t FULL = (R >= 64) ? ~0ull : ((1ull << R) - 1ull);
     if (R <= 32) {
-        out[i] = fast ? (uint64_t)three_edges_fast<uint32_t>(DD, nC, rB, dB, (float)R)
+        out[i] = fast ? (uint64_t)three_edges_fast<uint32_t>(DD, nC, rB, dB, Rs, R)
                       : (uint64_t)(three_edges<uint32_t, false>(e, DD, (float)R, R)) & FULL;
     } else {
-        out[i] = fast ? three_edges_fast<uint64_t>(DD, nC, rB, dB, (float)R)
+        out[i] = fast ? three_edges_fast<uint64_t>(DD, nC, rB, dB, Rs, R)
                       : three_edges<uint64_t, false>(e, DD, (float)R, R) & FULL;
     }
 }

@@ -159,6 +159,7 @@ void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm,
     // the early exit on full rows.
     a.chunkSlots = cdiv(a.WH, 64) + nb + 1;
     a.segs = (md.ntris > 0 && a.chunkSlots <= 2048) ? std::max(1, std::min(16, 16384 / a.chunkSlots)) : 1;
+    a.binLds = nb <= FD_LDS_BINS;
 }
 
 // Place the frame's workspace at `base` (nullptr: size only); returns bytes.
@@ -196,6 +197,7 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     const long np = std::max(std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)),
                              fdk::scan_partials_needed(T)) + 1;
     a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));
+    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), a.nblkT) * sizeof(uint32_t));
     a.opCount = countOps ? (unsigned long long*)take(FD_NOPS * sizeof(unsigned long long)) : nullptr;
     if (!a.masks) a.masks = (uint32_t*)take(WH * a.L.res * a.L.wpr * sizeof(uint32_t));
     return off;
@@ -581,6 +583,7 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
     for (int i = 0; i < n; ++i) {
         hArgs[i] = args[i];
         if (slots > 2048) hArgs[i].segs = 1;
+        hArgs[i].binLds = P.maxNb <= FD_LDS_BINS;   // the batched launch sizes LDS for the largest grid
     }
     int k = n;
     for (int cl = 0; cl < 2; ++cl)

@@ -55,6 +55,7 @@ struct FdArgs {
     int* triEnd;          // nb
     int* chunkStart;      // nb + 2
     uint64_t* scanPart;   // scan partial sums
+    uint32_t* part;       // per-block partials: 8 per k_reduce block (value-major), then 1 per pair-count block
     unsigned long long* opCount;  // optional occlusion counters (FD_NOPS), or null
     // occlusion work split: chunkSlots = receiver chunk slots (64 each, an upper
     // bound of the work items); each bin's triangle list is cut into segs
@@ -62,13 +63,20 @@ struct FdArgs {
     // masks merge by atomic OR into zeroed masks and a finishing pass counts them
     int chunkSlots;
     int segs;
+    // pair passes: bin statistics staged in LDS (the launch allocates
+    // FD_BIN_LDS_BYTES per bin besides the counters); set when nb <= FD_LDS_BINS
+    int binLds;
 };
+
+#define FD_LDS_BINS 1024
+#define FD_BIN_LDS_BYTES 20
 
 // Occlusion counters, in the order documented at fd_get_profile.
 #define FD_NOPS FD_PROFILE_COUNTERS
 
 // Stage boundaries recorded when profiling (events[0..FD_NSTAGES]).
-#define FD_NSTAGES 5   // light plane, receiver binning, triangle pairing, tables, occlusion
+#define FD_NSTAGES 5
+#define FD_FIN_THREADS 256   // k_finalize / k_post_pairs block size   // light plane, receiver binning, triangle pairing, tables, occlusion
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
 enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,

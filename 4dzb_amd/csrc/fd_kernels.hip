@@ -68,23 +68,25 @@ __device__ __forceinline__ void init_bins_body(const FdArgs& a, const int blk) {
     a.binStat[5 * b + 4] = encMax;     // min of rect max y
 }
 
-// Block-level reduction of NB uint32 values with op (0 = min, 1 = max),
-// then one atomic per block into dst.
+// Block-level reduction of NV uint32 values with op (0 = min, 1 = max,
+// 2 = sum); the block's results go to part[k * P + blk] (no atomics: many
+// blocks updating the few words of the frame state would serialise).
 template <int NV>
-__device__ void block_reduce_atomic(uint32_t (&v)[NV], const int (&op)[NV], uint32_t* const (&dst)[NV]) {
+__device__ void block_reduce_store(uint32_t (&v)[NV], const int (&op)[NV], uint32_t* part, int P, int blk) {
     __shared__ uint32_t sh[NV][RED_BLOCK / 64];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        uint32_t r = op[k] ? wave_max_u32(v[k]) : wave_min_u32(v[k]);
+        const uint32_t r = op[k] == 2 ? (uint32_t)wave_sum_i32((int)v[k]) : op[k] ? wave_max_u32(v[k]) : wave_min_u32(v[k]);
         if (l == 0) sh[k][w] = r;
     }
     __syncthreads();
     if (threadIdx.x < NV) {
         const int k = threadIdx.x;
         uint32_t r = sh[k][0];
-        for (int i = 1; i < RED_BLOCK / 64; ++i) r = op[k] ? max(r, sh[k][i]) : min(r, sh[k][i]);
-        if (op[k]) atomicMax(dst[k], r); else atomicMin(dst[k], r);
+        for (int i = 1; i < RED_BLOCK / 64; ++i)
+            r = op[k] == 2 ? r + sh[k][i] : op[k] ? max(r, sh[k][i]) : min(r, sh[k][i]);
+        part[(size_t)k * P + blk] = r;
     }
 }
 
@@ -138,13 +140,9 @@ __device__ __forceinline__ void reduce_body(const FdArgs& a, const int blk) {
             }
             a.rec[i] = out;
         }
-        FdState* s = a.st;
-        const int op[7] = {1, 0, 0, 0, 1, 1, 1};
-        uint32_t* const dst[7] = {&s->rxZmaxAll, &s->rxMin[0], &s->rxMin[1], &s->rxMin[2],
-                                  &s->rxMaxA[0], &s->rxMaxA[1], &s->rxMaxZ};
-        block_reduce_atomic<7>(v, op, dst);
-        cnt = wave_sum_i32(cnt);
-        if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s->validRect, cnt);
+        uint32_t v8[8] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], (uint32_t)cnt};
+        const int op[8] = {1, 0, 0, 0, 1, 1, 1, 2};
+        block_reduce_store<8>(v8, op, a.part, nbR + a.nbV, blk);
     } else {
         uint32_t v[7] = {encNegMax, encMax, encMax, encMax, encNegMax, encNegMax, encNegMax};
         const int base = (blk - nbR) * RED_BLOCK * RED_ITEMS + threadIdx.x;
@@ -164,11 +162,8 @@ __device__ __forceinline__ void reduce_body(const FdArgs& a, const int blk) {
             if (!(ax != ax)) v[4] = max(v[4], fenc(ax));
             if (!(ay != ay)) v[5] = max(v[5], fenc(ay));
         }
-        FdState* s = a.st;
         const int op[7] = {1, 0, 0, 0, 1, 1, 1};
-        uint32_t* const dst[7] = {&s->vxZmax, &s->vxMin[0], &s->vxMin[1], &s->vxMin[2],
-                                  &s->vxMaxA[0], &s->vxMaxA[1], &s->vxMaxZ};
-        block_reduce_atomic<7>(v, op, dst);
+        block_reduce_store<7>(v, op, a.part, nbR + a.nbV, blk);
     }
 }
 
@@ -177,8 +172,57 @@ __device__ __forceinline__ void reduce_body(const FdArgs& a, const int blk) {
 // (Step2_3_defineFDbuffer.inl:5-111) on the device, with its early exits.
 // --------------------------------------------------------------------------
 __device__ __forceinline__ void finalize_body(const FdArgs& a, const int blk) {
-    if (threadIdx.x != 0 || blk != 0) return;
+    if (blk != 0) return;
     FdState* s = a.st;
+    {
+        // the k_reduce block partials: receiver blocks [0, nbR), vertex
+        // blocks [nbR, nbR + nbV); min/max of order-preserving encodings
+        __shared__ uint32_t red[2][8][FD_FIN_THREADS / 64];
+        const int nbR = a.nbR, P = a.nbR + a.nbV;
+        const int op[8] = {1, 0, 0, 0, 1, 1, 1, 2};
+        const uint32_t encMax = fenc(FD_F32_MAX), encNegMax = fenc(-FD_F32_MAX);
+        uint32_t r[2][8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[0][k] = r[1][k] = op[k] == 2 ? 0u : op[k] ? encNegMax : encMax;
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            const int g = i < nbR ? 0 : 1;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (g == 1 && k == 7) continue;
+                const uint32_t x = a.part[(size_t)k * P + i];
+                r[g][k] = op[k] == 2 ? r[g][k] + x : op[k] ? max(r[g][k], x) : min(r[g][k], x);
+            }
+        }
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t x = op[k] == 2 ? (uint32_t)wave_sum_i32((int)r[g][k])
+                                              : op[k] ? wave_max_u32(r[g][k]) : wave_min_u32(r[g][k]);
+                if (l == 0) red[g][k][w] = x;
+            }
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        const int nw = (int)(blockDim.x + 63) >> 6;
+        uint32_t o[2][8];
+        for (int g = 0; g < 2; ++g)
+            for (int k = 0; k < 8; ++k) {
+                uint32_t x = red[g][k][0];
+                for (int j = 1; j < nw; ++j)
+                    x = op[k] == 2 ? x + red[g][k][j] : op[k] ? max(x, red[g][k][j]) : min(x, red[g][k][j]);
+                o[g][k] = x;
+            }
+        s->rxZmaxAll = o[0][0];
+        s->rxMin[0] = o[0][1]; s->rxMin[1] = o[0][2]; s->rxMin[2] = o[0][3];
+        s->rxMaxA[0] = o[0][4]; s->rxMaxA[1] = o[0][5];
+        s->rxMaxZ = o[0][6];
+        s->validRect = (int)o[0][7];
+        s->vxZmax = o[1][0];
+        s->vxMin[0] = o[1][1]; s->vxMin[1] = o[1][2]; s->vxMin[2] = o[1][3];
+        s->vxMaxA[0] = o[1][4]; s->vxMaxA[1] = o[1][5];
+        s->vxMaxZ = o[1][6];
+    }
     // sampleMaxZcal / triangleMaxZcal: thrust::reduce(max) seeded with -1
     const float zs = fmax_(-1.0f, fdec(s->rxZmaxAll));
     const float zt = fmax_(-1.0f, fdec(s->vxZmax));
@@ -263,10 +307,12 @@ __device__ __forceinline__ int tile_id(int p, int W, int H, int twl) {
 
 __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t hist[];  // nb + 1
+    __shared__ uint32_t blkCount[2];    // valid receivers, keys out of range
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
     for (int k = threadIdx.x; k <= nb; k += blockDim.x) hist[k] = 0;
+    if (threadIdx.x < 2) blkCount[threadIdx.x] = 0;
     __syncthreads();
     int nvalid = 0, noor = 0;
     const int base = blk * FD_RXB;
@@ -321,10 +367,14 @@ __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
     nvalid = wave_sum_i32(nvalid);
     noor = wave_sum_i32(noor);
     if (lane_id() == 0) {
-        if (nvalid) atomicAdd(&a.st->validSamples, nvalid);
-        if (noor) atomicAdd(&a.st->keyOutOfRange, noor);
+        if (nvalid) atomicAdd(&blkCount[0], (uint32_t)nvalid);
+        if (noor) atomicAdd(&blkCount[1], (uint32_t)noor);
     }
     __syncthreads();
+    // valid receivers = binned ones (the scanned histogram) + out-of-range
+    // keys, totalled in k_post_pairs; only the rare out-of-range count is
+    // accumulated here
+    if (threadIdx.x == 0 && blkCount[1]) atomicAdd(&a.st->keyOutOfRange, (int)blkCount[1]);
     for (int k = threadIdx.x; k <= nb; k += blockDim.x) a.histR[(size_t)k * a.nblkR + blk] = hist[k];
 }
 
@@ -472,6 +522,24 @@ __device__ __forceinline__ bool bin_test(const BinStat& b, float cbx, float cby,
     return b.maxZ >= cbz && cbx < b.mx && cby < b.my && b.mz < cex && b.mw < cey;
 }
 
+// Bin statistics staged in LDS by the pair passes (when the grid has at most
+// FD_LDS_BINS bins): float4 (maxZ, mx, my, mz) and mw per bin, decoded once
+// per block instead of five L2 loads per (triangle, bin) test.
+__device__ __forceinline__ void stage_bins(const FdArgs& a, int nb, float4* q, float* w) {
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        const uint32_t* bs = a.binStat + 5 * k;
+        q[k] = make_float4(fdec(bs[0]), fdec(bs[1]), fdec(bs[2]), fdec(bs[3]));
+        w[k] = fdec(bs[4]);
+    }
+}
+__device__ __forceinline__ BinStat bin_at(const FdArgs& a, bool staged, const float4* q, const float* w, int bp) {
+    if (staged) {
+        const float4 v = q[bp];
+        return BinStat{v.x, v.y, v.z, v.w, w[bp]};
+    }
+    return load_bin(a.binStat + 5 * bp);
+}
+
 struct TriClamp { float bx, by, bz, ex, ey; int sx, sy, ex_, ey_; bool ok; };
 
 // the count pass's triangle rejection, LP clamp and bin range (:41-83)
@@ -536,11 +604,17 @@ __device__ __forceinline__ void for_covered_bins(bool has, int sx, int sy, int e
 }
 
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
-    extern __shared__ uint32_t thist[];  // nb
+    extern __shared__ uint32_t thist[];  // nb counters, then the staged bin statistics
+    __shared__ uint32_t blkValid;
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
+    const bool staged = a.binLds && nb <= FD_LDS_BINS;
+    float4* binQ = (float4*)(thist + nb);
+    float* binW = (float*)(binQ + nb);
     for (int k = threadIdx.x; k < nb; k += blockDim.x) thist[k] = 0;
+    if (threadIdx.x == 0) blkValid = 0;
+    if (staged) stage_bins(a, nb, binQ, binW);
     __syncthreads();
     const int t = blk * FD_TRB + threadIdx.x;
     const bool act = t < a.ntris;
@@ -612,7 +686,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     }
     {
         const int nv = wave_sum_i32(valid);
-        if (lane_id() == 0 && nv) atomicAdd(&a.st->validTris, nv);
+        if (lane_id() == 0 && nv) atomicAdd(&blkValid, (uint32_t)nv);
     }
     // count pass
     TriClamp c = tri_clamp(mn, mx, s);
@@ -625,7 +699,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         for (int j = c.sy; j < c.ey_; ++j)
             for (int i = c.sx; i < c.ex_; ++i) {
                 const int bp = j << log2G | i;
-                const BinStat b = load_bin(a.binStat + 5 * bp);
+                const BinStat b = bin_at(a, staged, binQ, binW, bp);
                 if (bin_test(b, c.bx, c.by, c.bz, c.ex, c.ey)) {
                     ++count;
                     tsx = min(tsx, i); tsy = min(tsy, j);
@@ -640,6 +714,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
     }
     __syncthreads();
+    if (threadIdx.x == 0) a.part[blk] = blkValid;   // totalled in k_post_pairs
     for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blk] = thist[k];
 }
 
@@ -650,9 +725,17 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
 // order in which the reference's writeTriBinPair_kernel emits them
 // (Step4_RasterizeTriangleToBin.inl:161-204), with the count pass's test.
 __device__ __forceinline__ void tri_write_body(const FdArgs& a, const int blk) {
+    extern __shared__ uint32_t lds[];   // staged bin statistics
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
-    const int log2G = s->log2G;
+    const int nb = s->nb, log2G = s->log2G;
+    const bool staged = a.binLds && nb <= FD_LDS_BINS;
+    float4* binQ = (float4*)lds;
+    float* binW = (float*)(binQ + nb);
+    if (staged) {
+        stage_bins(a, nb, binQ, binW);
+        __syncthreads();
+    }
     const int t = blk * FD_TRB + threadIdx.x;
     if (t >= a.ntris) return;
     const uint2 rg = a.triRange[t];
@@ -666,7 +749,7 @@ __device__ __forceinline__ void tri_write_body(const FdArgs& a, const int blk) {
     for (int j = sy; j < ey; ++j)
         for (int i = sx; i < ex; ++i) {
             const int bp = j << log2G | i;
-            if (bin_test(load_bin(a.binStat + 5 * bp), c.bx, c.by, c.bz, c.ex, c.ey)) {
+            if (bin_test(bin_at(a, staged, binQ, binW, bp), c.bx, c.by, c.bz, c.ex, c.ey)) {
                 if (o < cap) a.pairKT[o] = tag | (uint32_t)bp;
                 ++o;
             }
@@ -771,9 +854,25 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
 // triangle reached the light plane, Step0_triangleSetup.inl:224-225;
 // FDLaunch skips the rest when there are no pairs, FD/api.cpp:522).
 __device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) {
-    if (threadIdx.x != 0 || blk != 0) return;
+    if (blk != 0) return;
     FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
+    // valid triangles: the pair-count blocks' totals (no count pass without
+    // triangles)
+    __shared__ int wsum[FD_FIN_THREADS / 64];
+    int v = 0;
+    if (a.ntris > 0)
+        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) v += (int)a.part[i];
+    v = wave_sum_i32(v);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int nv = 0;
+    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) nv += wsum[j];
+    s->validTris = nv;
+    // valid receivers: binned (keys < nb, the scanned histogram at key nb)
+    // plus keys out of range
+    s->validSamples = (int)a.histR[(size_t)s->nb * a.nblkR] + s->keyOutOfRange;
     if (s->validTris == 0) s->status = FD_EXIT_NO_VALID_TRI;
     else if (s->pairs == 0) s->status = FD_EXIT_NO_PAIRS;
     else if (s->pairs > a.pairCap || s->pairs >= 0x7fffffffull) { s->status = FD_EXIT_CAPACITY; s->capacityHit = 1; }
@@ -1441,14 +1540,14 @@ __device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c
 __global__ void k_init(FdArgs a) { init_body(a, blockIdx.x); }
 __global__ void k_init_bins(FdArgs a) { init_bins_body(a, blockIdx.x); }
 __global__ __launch_bounds__(RED_BLOCK) void k_reduce(FdArgs a) { reduce_body(a, blockIdx.x); }
-__global__ void k_finalize(FdArgs a) { finalize_body(a, blockIdx.x); }
+__global__ __launch_bounds__(FD_FIN_THREADS) void k_finalize(FdArgs a) { finalize_body(a, blockIdx.x); }
 __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) { rx_key_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) { rx_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) { tri_count_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_write(FdArgs a) { tri_write_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) { tri_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
-__global__ void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
+__global__ __launch_bounds__(FD_FIN_THREADS) void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
 // b % 8).  Work items of one bin share that bin's triangle list, so the
 // occlusion kernels renumber blocks so that each XCD takes runs of 16
@@ -1504,14 +1603,14 @@ __device__ __forceinline__ int inst_of(const int* __restrict__ map, int n, int b
 FD_BATCHED(kb_init, 64, init_body)
 FD_BATCHED(kb_init_bins, 256, init_bins_body)
 FD_BATCHED(kb_reduce, RED_BLOCK, reduce_body)
-FD_BATCHED(kb_finalize, 64, finalize_body)
+FD_BATCHED(kb_finalize, FD_FIN_THREADS, finalize_body)
 FD_BATCHED(kb_rx_key, 256, rx_key_body)
 FD_BATCHED(kb_rx_scatter, 64, rx_scatter_body)
 FD_BATCHED(kb_tri_count, FD_TRB, tri_count_body)
 FD_BATCHED(kb_tri_write, FD_TRB, tri_write_body)
 FD_BATCHED(kb_tri_scatter, 64, tri_scatter_body)
 FD_BATCHED(kb_tables, 1024, tables_body)
-FD_BATCHED(kb_post_pairs, 64, post_pairs_body)
+FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
 #undef FD_BATCHED
 
 template <typename T>
@@ -1635,27 +1734,28 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     const int n = p.n;
     auto L = [&](int m) { return dim3((unsigned)(p.total[m] > 0 ? p.total[m] : 1)); };
     const int nb = p.maxNb;
+    const size_t binLds = nb <= FD_LDS_BINS ? (size_t)nb * FD_BIN_LDS_BYTES : 0;
     kb_init<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     kb_init_bins<<<L(FD_MAP_BINS), 256, 0, st>>>(p.inst, p.map[FD_MAP_BINS], n);
     kb_reduce<<<L(FD_MAP_RED), RED_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_RED], n);
-    kb_finalize<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    kb_finalize<<<L(FD_MAP_ONE), FD_FIN_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     kb_rx_key<<<L(FD_MAP_RX), 256, (nb + 1) * sizeof(uint32_t), st>>>(p.inst, p.map[FD_MAP_RX], n);
     kb_scan_reduce<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
     kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 0);
     kb_scan_apply<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
     kb_rx_scatter<<<L(FD_MAP_RX), 64, (nb + 1) * sizeof(int), st>>>(p.inst, p.map[FD_MAP_RX], n);
     if (p.anyTris) {
-        kb_tri_count<<<L(FD_MAP_TRI), FD_TRB, nb * sizeof(uint32_t), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        kb_tri_count<<<L(FD_MAP_TRI), FD_TRB, nb * sizeof(uint32_t) + binLds, st>>>(p.inst, p.map[FD_MAP_TRI], n);
         kb_scan_reduce<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
         kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 1);
         kb_scan_apply<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
     }
-    kb_post_pairs<<<L(FD_MAP_ONE), 64, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    kb_post_pairs<<<L(FD_MAP_ONE), FD_FIN_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.anyTris) {
         kb_scan_reduce<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
         kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 2);
         kb_scan_apply<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
-        kb_tri_write<<<L(FD_MAP_TRI), FD_TRB, 0, st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        kb_tri_write<<<L(FD_MAP_TRI), FD_TRB, binLds, st>>>(p.inst, p.map[FD_MAP_TRI], n);
         kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
     }
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
@@ -1703,6 +1803,7 @@ hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st) {
 
 hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     const int nb = 1 << (2 * a.gridLog2);
+    const size_t binLds = a.binLds ? (size_t)nb * FD_BIN_LDS_BYTES : 0;
     const int nbR = a.nbR, nbV = a.nbV;
     FdState* s = a.st;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], st); };
@@ -1710,7 +1811,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     k_init<<<1, 64, 0, st>>>(a);
     k_init_bins<<<cdiv(nb, 256), 256, 0, st>>>(a);
     k_reduce<<<nbR + nbV, RED_BLOCK, 0, st>>>(a);
-    k_finalize<<<1, 64, 0, st>>>(a);
+    k_finalize<<<1, FD_FIN_THREADS, 0, st>>>(a);
     mark(1);
     // receivers: key + bin statistics, stable counting sort
     k_rx_key<<<a.nblkR, 256, (nb + 1) * sizeof(uint32_t), st>>>(a);
@@ -1720,16 +1821,16 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     mark(2);
     // triangles: setup + pair count, stable pair scatter
     if (a.ntris > 0) {
-        k_tri_count<<<a.nblkT, FD_TRB, nb * sizeof(uint32_t), st>>>(a);
+        k_tri_count<<<a.nblkT, FD_TRB, nb * sizeof(uint32_t) + binLds, st>>>(a);
         e = scan_u32(a.histT, a.nT, a.scanPart, &s->pairs, &s->status, st);
         if (e != hipSuccess) return e;
-        k_post_pairs<<<1, 64, 0, st>>>(a);
+        k_post_pairs<<<1, FD_FIN_THREADS, 0, st>>>(a);
         e = scan_u32(a.triOff, a.ntris, a.scanPart, &s->pairs, &s->status, st);
         if (e != hipSuccess) return e;
-        k_tri_write<<<a.nblkT, FD_TRB, 0, st>>>(a);
+        k_tri_write<<<a.nblkT, FD_TRB, binLds, st>>>(a);
         k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);
     } else {
-        k_post_pairs<<<1, 64, 0, st>>>(a);
+        k_post_pairs<<<1, FD_FIN_THREADS, 0, st>>>(a);
     }
     mark(3);
     k_tables<<<1, 1024, 0, st>>>(a);

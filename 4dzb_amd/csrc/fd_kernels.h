@@ -5,7 +5,7 @@
 #include "fd_types.h"
 
 #define FD_RXB 1024   // receivers per block in the key / scatter passes
-#define FD_TRB 256    // triangles per block in the count / scatter passes
+#define FD_TRB 1024   // triangles per block in the count / scatter passes
 
 // Everything one frame's kernels read, passed by value (kernarg segment).
 struct FdArgs {
@@ -75,8 +75,9 @@ struct FdArgs {
 #define FD_NOPS FD_PROFILE_COUNTERS
 
 // Stage boundaries recorded when profiling (events[0..FD_NSTAGES]).
-#define FD_NSTAGES 5
-#define FD_FIN_THREADS 256   // k_finalize / k_post_pairs block size   // light plane, receiver binning, triangle pairing, tables, occlusion
+#define FD_NSTAGES 5   // light plane, receiver binning, triangle pairing, tables, occlusion
+
+#define FD_FIN_THREADS 1024   // k_finalize / k_post_pairs block size
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
 enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,

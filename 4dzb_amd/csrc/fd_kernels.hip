@@ -23,7 +23,7 @@ using namespace fdk;
 namespace {
 
 constexpr int RED_BLOCK = 256;
-constexpr int RED_ITEMS = 4;           // receivers / vertices per thread in k_reduce
+constexpr int RED_ITEMS = 8;           // receivers / vertices per thread in k_reduce
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;
@@ -603,6 +603,31 @@ __device__ __forceinline__ void for_covered_bins(bool has, int sx, int sy, int e
     }
 }
 
+// Pair passes: a lane walks its own bin rectangle when it holds at most
+// FD_BIG_AREA bins; larger rectangles (a triangle spanning much of the light
+// plane) are walked by the whole wave, one rectangle at a time, so that one
+// lane's long walk does not hold up its wave and block.
+#define FD_BIG_AREA 64
+
+// The bins of a wave-uniform rectangle [sx, ex) x [sy, ey), 64 per step:
+// body(valid, i, j) on every lane.  Steps follow raster order and, within a
+// step, lanes do too (rows of up to 64 bins share a step when narrower).
+template <typename F>
+__device__ __forceinline__ void walk_rect_wave(int sx, int sy, int ex, int ey, F&& body) {
+    const int w = ex - sx;
+    const int lane = lane_id();
+    if (w <= 64) {
+        int wl = 0;
+        while ((1 << wl) < w) ++wl;
+        const int rows = 64 >> wl;
+        const int di = lane & ((1 << wl) - 1), dj = lane >> wl;
+        for (int j0 = sy; j0 < ey; j0 += rows) body(di < w && j0 + dj < ey, sx + di, j0 + dj);
+    } else {
+        for (int j = sy; j < ey; ++j)
+            for (int i0 = sx; i0 < ex; i0 += 64) body(i0 + lane < ex, i0 + lane, j);
+    }
+}
+
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t thist[];  // nb counters, then the staged bin statistics
     __shared__ uint32_t blkValid;
@@ -693,9 +718,10 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     if (!act) { c.ok = false; c.sx = c.sy = c.ex_ = c.ey_ = 0; }
     int count = 0, tsx = 6000, tsy = 6000, tex = 0, tey = 0;
     const bool has = c.ok && c.sx < c.ex_ && c.sy < c.ey_;
-    // every lane walks its own bin rectangle (bin statistics by vector loads
-    // from L2); per-block bin counts by LDS atomics
-    if (has) {
+    const bool big = has && (c.ex_ - c.sx) * (c.ey_ - c.sy) > FD_BIG_AREA;
+    // small rectangles: every lane walks its own; per-block bin counts by
+    // LDS atomics
+    if (has && !big) {
         for (int j = c.sy; j < c.ey_; ++j)
             for (int i = c.sx; i < c.ex_; ++i) {
                 const int bp = j << log2G | i;
@@ -708,6 +734,28 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
                 }
             }
     }
+    // large rectangles: the wave walks them one by one
+    for (uint64_t bigs = __ballot(big); bigs; bigs &= bigs - 1) {
+        const int L = __ffsll((unsigned long long)bigs) - 1;
+        const float bx = __shfl(c.bx, L, 64), by = __shfl(c.by, L, 64), bz = __shfl(c.bz, L, 64);
+        const float ex = __shfl(c.ex, L, 64), ey = __shfl(c.ey, L, 64);
+        const int sx = uniform(__shfl(c.sx, L, 64)), sy = uniform(__shfl(c.sy, L, 64));
+        const int ex_ = uniform(__shfl(c.ex_, L, 64)), ey_ = uniform(__shfl(c.ey_, L, 64));
+        int n = 0, mnx = 6000, mny = 6000, mxx = 0, mxy = 0;
+        walk_rect_wave(sx, sy, ex_, ey_, [&](bool v, int i, int j) {
+            const int bp = j << log2G | i;
+            const bool pass = v && bin_test(bin_at(a, staged, binQ, binW, v ? bp : 0), bx, by, bz, ex, ey);
+            if (pass) {
+                atomicAdd(&thist[bp], 1u);
+                mnx = min(mnx, i); mny = min(mny, j);
+                mxx = max(mxx, i + 1); mxy = max(mxy, j + 1);
+            }
+            n += __popcll(__ballot(pass));
+        });
+        mnx = wave_min_i32(mnx); mny = wave_min_i32(mny);
+        mxx = wave_max_i32(mxx); mxy = wave_max_i32(mxy);
+        if (lane_id() == L) { count = n; tsx = mnx; tsy = mny; tex = mxx; tey = mxy; }
+    }
     if (act) {
         a.triCount[t] = count;
         a.triOff[t] = (uint32_t)count;
@@ -716,6 +764,34 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     __syncthreads();
     if (threadIdx.x == 0) a.part[blk] = blkValid;   // totalled in k_post_pairs
     for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blk] = thist[k];
+}
+
+// The large rectangles of the write pass, walked by the whole wave; a pair's
+// place in its triangle's run is its rank among the passing bins in raster
+// order, as in the per-lane walk.
+__device__ __forceinline__ void tri_write_big(const FdArgs& a, int log2G, bool staged, const float4* binQ,
+                                              const float* binW, bool big, int sx, int sy, int ex_, int ey_,
+                                              const TriClamp& c, unsigned long long o0) {
+    const unsigned long long cap = a.pairCap;
+    for (uint64_t bigs = __ballot(big); bigs; bigs &= bigs - 1) {
+        const int L = __ffsll((unsigned long long)bigs) - 1;
+        const float bx = __shfl(c.bx, L, 64), by = __shfl(c.by, L, 64), bz = __shfl(c.bz, L, 64);
+        const float ex = __shfl(c.ex, L, 64), ey = __shfl(c.ey, L, 64);
+        const int rx0 = uniform(__shfl(sx, L, 64)), ry0 = uniform(__shfl(sy, L, 64));
+        const int rx1 = uniform(__shfl(ex_, L, 64)), ry1 = uniform(__shfl(ey_, L, 64));
+        unsigned long long o = __shfl(o0, L, 64);
+        const uint32_t tag = (uint32_t)((threadIdx.x & ~63u) + (unsigned)L) << 16;
+        walk_rect_wave(rx0, ry0, rx1, ry1, [&](bool v, int i, int j) {
+            const int bp = j << log2G | i;
+            const bool pass = v && bin_test(bin_at(a, staged, binQ, binW, v ? bp : 0), bx, by, bz, ex, ey);
+            const uint64_t m = __ballot(pass);
+            if (pass) {
+                const unsigned long long d = o + (unsigned)mbcnt(m);
+                if (d < cap) a.pairKT[d] = tag | (uint32_t)bp;
+            }
+            o += (unsigned)__popcll(m);
+        });
+    }
 }
 
 // Stable pair scatter (writeTriBinPair_kernel + sort_by_key,
@@ -737,23 +813,36 @@ __device__ __forceinline__ void tri_write_body(const FdArgs& a, const int blk) {
         __syncthreads();
     }
     const int t = blk * FD_TRB + threadIdx.x;
-    if (t >= a.ntris) return;
-    const uint2 rg = a.triRange[t];
-    const int sx = rg.x & 0xffff, sy = rg.x >> 16, ex = rg.y & 0xffff, ey = rg.y >> 16;
-    if (sx >= ex || sy >= ey) return;
-    const TriClamp c = tri_clamp(a.triAABB[2 * t], a.triAABB[2 * t + 1], s);
-    if (!c.ok) return;
-    unsigned long long o = a.triOff[t];
-    const unsigned long long cap = a.pairCap;
-    const uint32_t tag = (uint32_t)threadIdx.x << 16;
-    for (int j = sy; j < ey; ++j)
-        for (int i = sx; i < ex; ++i) {
-            const int bp = j << log2G | i;
-            if (bin_test(bin_at(a, staged, binQ, binW, bp), c.bx, c.by, c.bz, c.ex, c.ey)) {
-                if (o < cap) a.pairKT[o] = tag | (uint32_t)bp;
-                ++o;
-            }
+    if (uniform(blk * FD_TRB + (int)(threadIdx.x & ~63u)) >= a.ntris) return;   // whole wave past the end
+    // the walk over the pair range (bins with pairs) of the count pass
+    bool has = t < a.ntris;
+    int sx = 0, sy = 0, ex = 0, ey = 0;
+    TriClamp c{};
+    unsigned long long o = 0;
+    if (has) {
+        const uint2 rg = a.triRange[t];
+        sx = rg.x & 0xffff; sy = rg.x >> 16; ex = rg.y & 0xffff; ey = rg.y >> 16;
+        has = sx < ex && sy < ey;
+        if (has) {
+            c = tri_clamp(a.triAABB[2 * t], a.triAABB[2 * t + 1], s);
+            has = c.ok;
+            o = a.triOff[t];
         }
+    }
+    const bool big = has && (ex - sx) * (ey - sy) > FD_BIG_AREA;
+    if (has && !big) {
+        const unsigned long long cap = a.pairCap;
+        const uint32_t tag = (uint32_t)threadIdx.x << 16;
+        for (int j = sy; j < ey; ++j)
+            for (int i = sx; i < ex; ++i) {
+                const int bp = j << log2G | i;
+                if (bin_test(bin_at(a, staged, binQ, binW, bp), c.bx, c.by, c.bz, c.ex, c.ey)) {
+                    if (o < cap) a.pairKT[o] = tag | (uint32_t)bp;
+                    ++o;
+                }
+            }
+    }
+    tri_write_big(a, log2G, staged, binQ, binW, big, sx, sy, ex, ey, c, o);
 }
 
 // Stable counting scatter of one triangle block's pairs (the reference's

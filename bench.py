@@ -337,6 +337,17 @@ def gpu_target_config(pkg, steps=10):
     fd, wl = pkg.fd4d, pkg.workloads
     frame = wl.boxes_1080p(light_res=32)
     strm = fd.Stream()
+    # layout (bins per side x receiver tile width) from the autotuner: the
+    # fastest candidate whose every output bit equals the default layout's
+    run = fd.FrameRunner(frame, stream=strm)
+    item = dict(pos=run.pos, nrm=run.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=run.shadow,
+                max_pairs=64 << 20)
+    cands = pkg.autotune.CANDIDATES
+    best, _ = run.ctx.autotune(item, cands, 3)
+    bins, tile = cands[best] if best >= 0 else (16, 8)
+    run.close()
+    frame.grid_log2 = bins.bit_length() - 1
+    frame.tile_width = tile
     run = fd.FrameRunner(frame, stream=strm)
     run.run()
     lat = []
@@ -346,6 +357,7 @@ def gpu_target_config(pkg, steps=10):
         lat.append((time.perf_counter() - a) * 1e3)
     run.close()
     return dict(workload="boxes-1080p with a 32x32 light (1920x1080 receivers, 1M triangles)",
+                layout=dict(bins=bins, tile=tile), frames=steps,
                 p50_ms=round(pctl(lat, 50), 2), p99_ms=round(pctl(lat, 99), 2), exit=st.exit_code,
                 gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
@@ -452,7 +464,7 @@ def main():
         log("[bench] serving form (serve-poisson)")
         out["served"] = gpu_served(pkg)
         log("[bench] latency target configuration")
-        out["target_1080p_32x32_1M"] = gpu_target_config(pkg)
+        out["target_1080p_32x32_1M"] = gpu_target_config(pkg, steps=100)
     if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)

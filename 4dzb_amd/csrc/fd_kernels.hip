@@ -1512,28 +1512,36 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             }
         };
         if (!__any(big)) {
-            // fast form: finite limits in every lane of the wave
-            const T win = row_window<T>(e, Rf, mAB);
-            if (!__any(win != (T)0)) continue;   // no lane can gain a bit from this triangle
-            if (!COUNT && onTri && win != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+            // fast form: finite limits in every lane of the wave.
             // |A|, |B| <= 2^100 here, so only the lower end of the reciprocal
             // range is checked (NaN coefficients fail it as well);
             // DN = C + max(B (R-1), 0) equals (B > 0 ? B (R-1) : 0) + C up to
             // the sign of a zero, B being finite or NaN
-            float nR[3], rA[3], dA[3], nC[3], rB[3], dB[3], DN[3];
+            float nR[3], rA[3], dA[3], bn[3], DN[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 dA[k] = e[k].A;
-                dB[k] = e[k].B;
                 nR[k] = -(e[k].A * nsub);
-                nC[k] = -(e[k].B * nsub);
+                bn[k] = e[k].B * nsub;
                 rA[k] = fabsf(e[k].A) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].A) : __builtin_nanf("");
+                DN[k] = e[k].C + fmaxf(bn[k], 0.0f);
+            }
+            // rows that may hold occluded samples (:176-191), then the row
+            // window; a wave none of whose lanes keeps a row skips the
+            // triangle at either step (the cheaper test first)
+            T valid = (~full) & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
+            if (!__any(valid != (T)0)) continue;
+            valid &= row_window<T>(e, Rf, mAB);
+            if (!__any(valid != (T)0)) continue;   // no lane can gain a bit from this triangle
+            if (!COUNT && onTri && valid != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+            float nC[3], rB[3], dB[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                dB[k] = e[k].B;
+                nC[k] = -bn[k];
                 rB[k] = fabsf(e[k].B) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].B) : __builtin_nanf("");
                 asm("" : "+v"(rB[k]));   // keep the select out of the row loop
-                DN[k] = e[k].C + fmaxf(-nC[k], 0.0f);
             }
-            // rows that may hold occluded samples (:176-191)
-            const T valid = (~full) & win & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
             wave_rows(valid);
             rows(valid, [&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rs, R); });
         } else {

@@ -410,6 +410,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--light-res", type=int, default=0, help="override the workload's light grid side")
+    ap.add_argument("--bins", type=int, default=0, help="bins per light-plane side (default: the workload's, 16)")
+    ap.add_argument("--tile", type=int, default=0, help="receiver tile width (default 8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
@@ -424,6 +426,10 @@ def main():
     make = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
         args.workload]
     frame = make(light_res=args.light_res) if args.light_res else make()
+    if args.bins:
+        frame.grid_log2 = args.bins.bit_length() - 1
+    if args.tile:
+        frame.tile_width = args.tile
     log("[bench] GPU plain path")
     g = gpu_plain(pkg, frame, args.steps, args.warmup, args.quick)
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
@@ -450,7 +456,7 @@ def main():
                dtype="f32", data="synthetic (seeded generators, 4dzb_amd/csrc/scenes.cpp)",
                config=dict(workload=args.workload, receivers=f"{W}x{H}", light_samples=f"{R}x{R}",
                            triangles=frame.scene.ntris, bins=f"{1 << frame.grid_log2}x{1 << frame.grid_log2}",
-                           path="plain per-frame"),
+                           tile_width=frame.tile_width, path="plain per-frame"),
                p50_ms=round(pctl(g["lat"], 50), 3), p99_ms=round(pctl(g["lat"], 99), 3),
                stage_ms=dict(zip(["light_plane", "receiver_binning", "triangle_pairing", "tables", "occlusion"],
                                  [round(x, 4) for x in g["stage"]])),

@@ -135,3 +135,30 @@ def test_repeat_launch_idempotent(fd, oracle, wl):
     rep2, _, _ = compare(fd, oracle, fr, fr=run, o=o)
     assert not failures(rep2), failures(rep2)
     run.close()
+
+
+@pytest.mark.parametrize("grid_log2", [5, 6])
+def test_large_triangles_wave_walk(fd, oracle, wl, grid_log2):
+    """Triangles whose bin rectangles exceed 64 bins are walked by the whole
+    wave in the pair count and write passes; mixed with small ones so that
+    both walks share waves.  Pair lists, counts and ranges stay bit-exact."""
+    fr = wl.plane_256(ntri=1500, res=128)
+    big = []
+    for k, z in enumerate((0.8, 1.2, 1.6, 2.0, 2.4)):
+        s = 3.2 - 0.4 * k
+        big += [(-s, -s, z), (s, -s + 0.1 * k, z), (0.1 * k, s, z)]      # counter-clockwise from +z
+    big += [(-1.5, -0.5, 1.0), (1.5, -0.4, 1.1), (0.2, 1.4, 1.05)]
+    bv = np.array([(x, y, zz, 1.0) for x, y, zz in big], np.float32)
+    n0 = fr.scene.nverts
+    fr.scene.verts = np.concatenate([fr.scene.verts, bv])
+    bi = (n0 + np.arange(len(big), dtype=np.int32)).reshape(-1, 3)
+    # interleave the large triangles among the small ones (different waves and lanes)
+    idx = fr.scene.idx
+    fr.scene.idx = np.concatenate([idx[:40], bi[:2], idx[40:700], bi[2:4], idx[700:], bi[4:]]).astype(np.int32)
+    fr.grid_log2 = grid_log2
+    o = _check(fd, oracle, fr)
+    assert o.exit == 0
+    tr = o.tri_range
+    w = tr["ex"].astype(np.int64) - tr["sx"]
+    h = tr["ey"].astype(np.int64) - tr["sy"]
+    assert int(((w > 0) & (h > 0) & (w * h > 64)).sum()) >= 3, "the frame must hold triangles with more than 64 bins"

@@ -103,24 +103,41 @@ int fdgen_receivers_heightfield(int w, int h, float hx, float hy, float amp, flo
     return 0;
 }
 
-// A G-buffer ray-cast on the host: a pinhole camera over a ground plane
-// (|x|,|y| <= 10) with `nbox` axis-aligned boxes standing on it.  Background
-// pixels get pos = NaN.  Gives depth discontinuities and back-facing walls.
-int fdgen_receivers_boxes(int w, int h, int nbox, uint64_t seed, float* pos4, float* nrm4) {
-    if (w <= 0 || h <= 0 || nbox < 0 || nbox > 256 || !pos4 || !nrm4) return -1;
+// Box layout of the boxes G-buffer: `nbox` axis-aligned boxes standing on
+// the ground, centre U[-7,7]^2, half extents U[0.3,1.0], height U[0.5,2.5].
+// out: nbox x {min x, y, z, max x, y, z}.
+int fdgen_boxes_layout(int nbox, uint64_t seed, double* out) {
+    if (nbox < 0 || nbox > 256 || (nbox > 0 && !out)) return -1;
     Rng r(seed);
-    double bmin[256][3], bmax[256][3];
     for (int b = 0; b < nbox; ++b) {
         double cx = r.uni(-7, 7), cy = r.uni(-7, 7);
         double sx = r.uni(0.3, 1.0), sy = r.uni(0.3, 1.0), hz = r.uni(0.5, 2.5);
-        bmin[b][0] = cx - sx; bmin[b][1] = cy - sy; bmin[b][2] = 0.0;
-        bmax[b][0] = cx + sx; bmax[b][1] = cy + sy; bmax[b][2] = hz;
+        double* o = out + 6 * b;
+        o[0] = cx - sx; o[1] = cy - sy; o[2] = 0.0;
+        o[3] = cx + sx; o[4] = cy + sy; o[5] = hz;
     }
-    const D3 eye{0.0, -13.0, 9.0}, at{0.0, 0.0, 0.0}, upv{0.0, 0.0, 1.0};
+    return 0;
+}
+
+// A G-buffer ray-cast on the host: a pinhole camera (eye, at, up, vertical
+// field of view) over the ground plane z = 0 (|x|,|y| <= ground_half) with
+// `nbox` axis-aligned boxes standing on it.  Writes the four RGBA32F targets
+// of the reference's G-buffer pass (gatherData.shf:41-44): g0 = (position,
+// ks.r), g1 = (normal, ks.g), g2 = (ka, ks.b), g3 = (kd, shininess), with
+// mat = nbox + 1 materials of 10 floats (ka, kd, ks, shininess; [0] the
+// ground).  Background texels are NaN in every channel (the reference clears
+// to NaN, Render.cpp:423-426).  g2 / g3 / mat may be null (then only g0, g1
+// are written, with w = 1).  Double precision, in a fixed operation order
+// (the device ray cast repeats it bit for bit).
+int fdgen_gbuffer_boxes(int w, int h, int nbox, const double* boxes, const double* eye3, const double* at3,
+                        const double* up3, double fovy_deg, double ground_half, const float* mat, float* g0, float* g1,
+                        float* g2, float* g3) {
+    if (w <= 0 || h <= 0 || nbox < 0 || nbox > 256 || (nbox > 0 && !boxes) || !g0 || !g1) return -1;
+    const D3 eye{eye3[0], eye3[1], eye3[2]}, at{at3[0], at3[1], at3[2]}, upv{up3[0], up3[1], up3[2]};
     D3 f = normalize(at - eye);
     D3 s = normalize(cross(f, upv));
     D3 u = cross(s, f);
-    double tanh_ = std::tan(0.5 * 50.0 * 3.141592653589793 / 180.0);
+    double tanh_ = std::tan(0.5 * fovy_deg * 3.141592653589793 / 180.0);
     double aspect = (double)w / h;
     const float qnan = std::numeric_limits<float>::quiet_NaN();
     for (int y = 0; y < h; ++y)
@@ -131,23 +148,26 @@ int fdgen_receivers_boxes(int w, int h, int nbox, uint64_t seed, float* pos4, fl
             D3 d = normalize(f + s * ndcx + u * ndcy);
             double tbest = 1e300;
             D3 nbest{0, 0, 0};
+            int obj = -1;
             if (d.z < 0.0) {
                 double t = -eye.z / d.z;
                 D3 p = eye + d * t;
-                if (std::fabs(p.x) <= 10.0 && std::fabs(p.y) <= 10.0) { tbest = t; nbest = {0, 0, 1}; }
+                if (std::fabs(p.x) <= ground_half && std::fabs(p.y) <= ground_half) { tbest = t; nbest = {0, 0, 1}; obj = 0; }
             }
             const double e[3] = {eye.x, eye.y, eye.z}, dd[3] = {d.x, d.y, d.z};
             for (int b = 0; b < nbox; ++b) {
+                const double* bmin = boxes + 6 * b;
+                const double* bmax = bmin + 3;
                 double t0 = -1e300, t1 = 1e300;
                 int axis = -1;
                 double sgn = 0;
                 bool miss = false;
                 for (int a = 0; a < 3; ++a) {
                     if (std::fabs(dd[a]) < 1e-300) {
-                        if (e[a] < bmin[b][a] || e[a] > bmax[b][a]) { miss = true; break; }
+                        if (e[a] < bmin[a] || e[a] > bmax[a]) { miss = true; break; }
                         continue;
                     }
-                    double ta = (bmin[b][a] - e[a]) / dd[a], tb = (bmax[b][a] - e[a]) / dd[a];
+                    double ta = (bmin[a] - e[a]) / dd[a], tb = (bmax[a] - e[a]) / dd[a];
                     double sa = -1.0;
                     if (ta > tb) { double tmp = ta; ta = tb; tb = tmp; sa = 1.0; }
                     if (ta > t0) { t0 = ta; axis = a; sgn = sa; }
@@ -158,16 +178,39 @@ int fdgen_receivers_boxes(int w, int h, int nbox, uint64_t seed, float* pos4, fl
                     tbest = t0;
                     nbest = {0, 0, 0};
                     if (axis == 0) nbest.x = sgn; else if (axis == 1) nbest.y = sgn; else nbest.z = sgn;
+                    obj = 1 + b;
                 }
             }
             if (tbest < 1e299) {
-                put4(pos4 + 4 * i, eye + d * tbest);
-                put4(nrm4 + 4 * i, nbest);
+                const float* m = mat ? mat + 10 * obj : nullptr;
+                put4(g0 + 4 * i, eye + d * tbest);
+                put4(g1 + 4 * i, nbest);
+                if (m) {
+                    g0[4 * i + 3] = m[6];
+                    g1[4 * i + 3] = m[7];
+                    if (g2) { g2[4 * i] = m[0]; g2[4 * i + 1] = m[1]; g2[4 * i + 2] = m[2]; g2[4 * i + 3] = m[8]; }
+                    if (g3) { g3[4 * i] = m[3]; g3[4 * i + 1] = m[4]; g3[4 * i + 2] = m[5]; g3[4 * i + 3] = m[9]; }
+                }
             } else {
-                for (int k = 0; k < 4; ++k) { pos4[4 * i + k] = qnan; nrm4[4 * i + k] = qnan; }
+                for (int k = 0; k < 4; ++k) {
+                    g0[4 * i + k] = qnan; g1[4 * i + k] = qnan;
+                    if (g2) g2[4 * i + k] = qnan;
+                    if (g3) g3[4 * i + k] = qnan;
+                }
             }
         }
     return 0;
+}
+
+// The boxes receivers of the workloads: the G-buffer above with 20 boxes by
+// default, camera at (0, -13, 9) looking at the origin, 50 degrees, ground
+// half-size 10; position and normal only (w = 1).
+int fdgen_receivers_boxes(int w, int h, int nbox, uint64_t seed, float* pos4, float* nrm4) {
+    if (w <= 0 || h <= 0 || nbox < 0 || nbox > 256 || !pos4 || !nrm4) return -1;
+    double boxes[256 * 6];
+    fdgen_boxes_layout(nbox, seed, boxes);
+    const double eye[3] = {0.0, -13.0, 9.0}, at[3] = {0.0, 0.0, 0.0}, up[3] = {0.0, 0.0, 1.0};
+    return fdgen_gbuffer_boxes(w, h, nbox, boxes, eye, at, up, 50.0, 10.0, nullptr, pos4, nrm4, nullptr, nullptr);
 }
 
 // `ntri` unshared equilateral triangles: centroid uniform in the box, edge

@@ -77,6 +77,24 @@ class Request(ctypes.Structure):
                 ("d_shadow", ctypes.c_void_p), ("d_masks", ctypes.c_void_p), ("user_tag", ctypes.c_ulonglong)]
 
 
+class Material(ctypes.Structure):
+    _fields_ = [("ka", ctypes.c_float * 3), ("kd", ctypes.c_float * 3), ("ks", ctypes.c_float * 3),
+                ("shininess", ctypes.c_float)]
+
+
+class GbufferScene(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("nbox", ctypes.c_int), ("pad", ctypes.c_int),
+                ("boxes", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("eye", ctypes.c_double * 3),
+                ("at", ctypes.c_double * 3), ("up", ctypes.c_double * 3), ("fovy_deg", ctypes.c_double),
+                ("ground_half", ctypes.c_double)]
+
+
+class ShadeParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("nsamples", ctypes.c_int), ("pad", ctypes.c_int),
+                ("samples", ctypes.c_void_p), ("camera", ctypes.c_float * 3), ("La", ctypes.c_float * 3),
+                ("Ld", ctypes.c_float * 3), ("Ls", ctypes.c_float * 3)]
+
+
 class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_longlong), ("user_tag", ctypes.c_ulonglong), ("status", ctypes.c_int),
                 ("exit_code", ctypes.c_int), ("batch_size", ctypes.c_int), ("pad", ctypes.c_int),
@@ -118,6 +136,8 @@ def lib():
             "fd_set_pair_capacity": ([vp, sz], ip),
             "fd_move_model": ([vp, ctypes.POINTER(ctypes.c_float), vp], ip),
             "fd_mask_words": ([ip], ip),
+            "fd_gbuffer_boxes": ([ctypes.POINTER(GbufferScene), vp, vp, vp, vp, vp], ip),
+            "fd_shade": ([ctypes.POINTER(ShadeParams), vp, vp, vp, vp, vp, vp, vp], ip),
             "fd_debug_read": ([vp, ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)], ip),
             "fd_status_string": ([ip], ctypes.c_char_p),
             "fd_set_profiling": ([vp, ip, ip], ip),
@@ -526,6 +546,34 @@ LIGHT_DTYPE = np.dtype([("m", "<f4", 16), ("xd", "<f4", 3), ("yd", "<f4", 3), ("
 TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), ("bminx", "<f4"), ("bminy", "<f4"),
                       ("bmaxx", "<f4"), ("bmaxy", "<f4"), ("minz", "<f4"), ("maxz", "<f4"), ("pad0", "<f4"),
                       ("n", "<f4", 3), ("e01", "<f4", 3), ("e12", "<f4", 3), ("e20", "<f4", 3), ("pad1", "<f4", 4)])
+
+
+def gbuffer_boxes(width, height, boxes, materials, eye, at, up, fovy_deg, ground_half, g0, g1, g2=None, g3=None,
+                  stream=None):
+    """fd_gbuffer_boxes: boxes (nbox, 6) float64, materials (nbox + 1, 10)
+    float32 (ka, kd, ks, shininess) or None; g0..g3 DeviceBuffers of
+    width*height float4 (g2, g3 optional)."""
+    import numpy as np
+    bx = np.ascontiguousarray(np.asarray(boxes, np.float64).reshape(-1, 6))
+    mt = None if materials is None else np.ascontiguousarray(np.asarray(materials, np.float32).reshape(-1, 10))
+    sc = GbufferScene(width, height, bx.shape[0], 0, bx.ctypes.data if bx.size else None,
+                      mt.ctypes.data if mt is not None else None, (ctypes.c_double * 3)(*eye),
+                      (ctypes.c_double * 3)(*at), (ctypes.c_double * 3)(*up), fovy_deg, ground_half)
+    h = stream.handle if isinstance(stream, Stream) else stream
+    _check(lib().fd_gbuffer_boxes(ctypes.byref(sc), _ptr(g0), _ptr(g1), _ptr(g2) if g2 is not None else None,
+                                  _ptr(g3) if g3 is not None else None, h), "fd_gbuffer_boxes")
+
+
+def shade(width, height, samples, camera, La, Ld, Ls, shadow, g0, g1, g2, g3, rgba, stream=None):
+    """fd_shade: samples (ns, 3) world positions; shadow / g0..g3 / rgba
+    DeviceBuffers (float x W*H, float4 x W*H)."""
+    import numpy as np
+    sp = np.ascontiguousarray(np.asarray(samples, np.float32).reshape(-1, 3))
+    f3 = lambda v: (ctypes.c_float * 3)(*[float(x) for x in v])
+    p = ShadeParams(width, height, sp.shape[0], 0, sp.ctypes.data, f3(camera), f3(La), f3(Ld), f3(Ls))
+    h = stream.handle if isinstance(stream, Stream) else stream
+    _check(lib().fd_shade(ctypes.byref(p), _ptr(shadow), _ptr(g0), _ptr(g1), _ptr(g2), _ptr(g3), _ptr(rgba), h),
+           "fd_shade")
 
 
 def read_state(ctx):

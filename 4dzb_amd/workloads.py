@@ -55,6 +55,10 @@ def _scenes():
         lib.fdgen_spheres.restype = ctypes.c_long
         lib.fdgen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_long, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_double)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.fdgen_boxes_layout.argtypes = [ctypes.c_int, ctypes.c_uint64, dp]
+        lib.fdgen_gbuffer_boxes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, ctypes.c_double,
+                                            ctypes.c_double, fp, fp, fp, fp, fp]
         _lib = lib
     return _lib
 
@@ -244,3 +248,54 @@ def poisson_requests(n, rate_hz=500.0, nscenes=4, seed=5, res_choices=BATCH_RES,
         pose = (-2 + 4 * u[i, 4], -2 + 4 * u[i, 5], 4 + 4 * u[i, 6])
         reqs.append(dict(t=float(t[i]), scene=sc, w=w, h=h, grid=g, pose=tuple(float(x) for x in pose)))
     return reqs
+
+
+@dataclass
+class GbufferScene:
+    """The boxes G-buffer scene (ground z = 0 + axis-aligned boxes, pinhole
+    camera) as fd_gbuffer_boxes takes it."""
+    width: int
+    height: int
+    boxes: np.ndarray       # (nbox, 6) float64: min xyz, max xyz
+    materials: np.ndarray   # (nbox + 1, 10) float32: ka, kd, ks, shininess ([0] the ground)
+    eye: tuple = (0.0, -13.0, 9.0)
+    at: tuple = (0.0, 0.0, 0.0)
+    up: tuple = (0.0, 0.0, 1.0)
+    fovy_deg: float = 50.0
+    ground_half: float = 10.0
+
+
+def boxes_materials(nbox):
+    """Ground: grey, weak specular; box b: a hue around the colour wheel."""
+    m = np.zeros((nbox + 1, 10), np.float32)
+    m[0] = (0.08, 0.08, 0.08, 0.55, 0.55, 0.5, 0.15, 0.15, 0.15, 8.0)
+    for b in range(nbox):
+        t = 6.283185307179586 * b / max(nbox, 1)
+        kd = 0.45 + 0.35 * np.cos(t + np.array([0.0, 2.0943951, 4.1887902]))
+        m[1 + b, 0:3] = 0.1 * kd
+        m[1 + b, 3:6] = kd
+        m[1 + b, 6:9] = 0.35
+        m[1 + b, 9] = 24.0 + 8.0 * (b % 4)
+    return m
+
+
+def gbuffer_scene_boxes(w=1920, h=1080, nbox=20, seed=3):
+    """The scene of receivers_boxes (same boxes, camera, ground) with materials."""
+    boxes = np.zeros((nbox, 6), np.float64)
+    assert _scenes().fdgen_boxes_layout(nbox, seed, boxes.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    return GbufferScene(w, h, boxes, boxes_materials(nbox))
+
+
+def gbuffer_boxes_host(sc):
+    """Host ray cast of the same scene (the device producer's reference):
+    four (H, W, 4) float32 targets g0..g3."""
+    H, W = sc.height, sc.width
+    g = [np.empty((H, W, 4), np.float32) for _ in range(4)]
+    dp = lambda a: np.ascontiguousarray(np.asarray(a, np.float64)).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    bx = np.ascontiguousarray(sc.boxes, np.float64)
+    mt = np.ascontiguousarray(sc.materials, np.float32)
+    eye, at, up = (np.asarray(v, np.float64) for v in (sc.eye, sc.at, sc.up))
+    rc = _scenes().fdgen_gbuffer_boxes(W, H, bx.shape[0], dp(bx), dp(eye), dp(at), dp(up), sc.fovy_deg,
+                                       sc.ground_half, _fp(mt), _fp(g[0]), _fp(g[1]), _fp(g[2]), _fp(g[3]))
+    assert rc == 0
+    return g

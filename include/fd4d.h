@@ -284,6 +284,59 @@ int fd_stream_destroy(void* hip_stream);
 int fd_stream_sync(void* hip_stream);
 int fd_device_sync(void);
 
+/* ---- Application stages on either side of the visibility path --------- */
+
+/* One object's material, as the reference's G-buffer pass packs it
+ * (Resources/shader/gatherData.shf:25-44): ka, kd, ks (rgb), shininess. */
+typedef struct fd_material {
+    float ka[3], kd[3], ks[3], shininess;
+} fd_material;
+
+/* A ground plane z = 0 (|x|, |y| <= ground_half) with nbox axis-aligned boxes
+ * standing on it, seen by a pinhole camera (eye, at, up; vertical field of
+ * view in degrees).  materials: nbox + 1 entries, [0] the ground's, [1 + b]
+ * box b's (may be NULL when g2 / g3 are not requested). */
+typedef struct fd_gbuffer_scene {
+    int width, height;
+    int nbox, pad;
+    const double* boxes;          /* host: nbox x {min x, y, z, max x, y, z} */
+    const fd_material* materials; /* host */
+    double eye[3], at[3], up[3];
+    double fovy_deg, ground_half;
+} fd_gbuffer_scene;
+
+/* Replaces Render::pass0 + gatherData.shf (4DZB/Render/Render.cpp:390-431,
+ * 4DZB/Resources/shader/gatherData.shf:22-46): ray-casts the scene into the
+ * four RGBA32F G-buffer targets (float4 x width*height, row-major y*W + x):
+ * g0 = (position, ks.r), g1 = (normal, ks.g), g2 = (ka, ks.b),
+ * g3 = (kd, shininess).  Background texels are NaN in every channel (the
+ * reference clears to NaN, Render.cpp:423-426).  g0 / g1 are the visibility
+ * path's receiver buffers (fd_set_sample_buffers); g2 / g3 may be NULL
+ * (then, without materials, g0.w = g1.w = 1).  Double-precision ray cast,
+ * stream-ordered on hip_stream (no host synchronisation). */
+int fd_gbuffer_boxes(const fd_gbuffer_scene* scene, float* d_g0, float* d_g1, float* d_g2, float* d_g3,
+                     void* hip_stream);
+
+/* Light terms of the shading pass (phoneShading.shf:12-22). */
+typedef struct fd_shade_params {
+    int width, height;
+    int nsamples, pad;      /* light sample positions (the reference binds 1, Render.cpp:550) */
+    const float* samples;   /* host: nsamples x 3 world positions, in accumulation order */
+    float camera[3];
+    float La[3], Ld[3], Ls[3];
+} fd_shade_params;
+
+/* Replaces Render::pass2 + phoneShading.shf (4DZB/Render/Render.cpp:510-565,
+ * 4DZB/Resources/shader/phoneShading.shf:23-61): per texel,
+ * intensity = 1 - shadow (the visibility output); a texel whose intensity is
+ * NaN keeps the cleared colour (0, 0, 0, 0); otherwise the Phong terms of
+ * each light sample, ambient + (diffuse + specular) * intensity, summed and
+ * divided by the sample count.  Float arithmetic with IEEE division and
+ * square root (normalize(v) = v / sqrt(dot(v, v)), pow = powf).  Writes
+ * float4 x width*height; stream-ordered. */
+int fd_shade(const fd_shade_params* params, const float* d_shadow, const float* d_g0, const float* d_g1,
+             const float* d_g2, const float* d_g3, float* d_rgba, void* hip_stream);
+
 /* Words per receiver in the mask output for a res x res light grid. */
 int fd_mask_words(int light_res);
 

@@ -895,6 +895,57 @@ void or_make_rectangle(const float* mat16, const float* p3, float* out3) {
     out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
 }
 
+// The shading pass of the reference application (deferred Phong over the
+// light samples, 4DZB/Resources/shader/phoneShading.shf:23-61), in float:
+// normalize(v) = v / sqrt(dot(v, v)), pow = powf, max = fmaxf, expressions in
+// the shader's order.  g0..g3: float4 x n (gatherData.shf:41-44 packing);
+// samples: ns x 3; light: La, Ld, Ls (3 each); cam: 3.  out: float4 x n; a
+// texel whose intensity 1 - shadow is NaN keeps (0, 0, 0, 0) (discard).
+void or_shade(int n, const float* shadow, const float* g0, const float* g1, const float* g2, const float* g3,
+              int ns, const float* samples, const float* cam, const float* La, const float* Ld, const float* Ls,
+              float* out) {
+    auto dot = [](const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    auto normalize = [&](float* v) {
+        const float l = std::sqrt(dot(v, v));
+        v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
+    };
+    for (int i = 0; i < n; ++i) {
+        float* o = out + 4 * (size_t)i;
+        const float intensity = 1.0f - shadow[i];                  // :25
+        if (std::isnan(intensity)) { o[0] = o[1] = o[2] = o[3] = 0.0f; continue; }   // :26-27
+        const float* a0 = g0 + 4 * (size_t)i;
+        const float* a1 = g1 + 4 * (size_t)i;
+        const float* a2 = g2 + 4 * (size_t)i;
+        const float* a3 = g3 + 4 * (size_t)i;
+        const float P[3] = {a0[0], a0[1], a0[2]};                  // :34-39
+        const float ka[3] = {a2[0], a2[1], a2[2]}, kd[3] = {a3[0], a3[1], a3[2]}, ks[3] = {a0[3], a1[3], a2[3]};
+        const float shin = a3[3];
+        float N[3] = {a1[0], a1[1], a1[2]};
+        normalize(N);                                               // :41
+        float V[3] = {cam[0] - P[0], cam[1] - P[1], cam[2] - P[2]};
+        normalize(V);                                               // :42
+        float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < ns; ++k) {                              // :45-58
+            float L[3] = {samples[3 * k] - P[0], samples[3 * k + 1] - P[1], samples[3 * k + 2] - P[2]};
+            normalize(L);
+            const float LdN = dot(L, N);
+            const float t2 = 2.0f * LdN;
+            const float R[3] = {t2 * N[0] - L[0], t2 * N[1] - L[1], t2 * N[2] - L[2]};
+            const float dif = std::fmax(LdN, 0.0f);
+            const float spw = std::pow(std::fmax(dot(R, V), 0.0f), shin);
+            for (int j = 0; j < 3; ++j) {
+                const float amb = La[j] * ka[j];
+                const float dd = Ld[j] * kd[j] * dif;
+                const float sc = Ls[j] * ks[j] * spw;
+                c[j] += amb + (dd + sc) * intensity;
+            }
+            c[3] += 1.0f;
+        }
+        const float cnt = (float)(unsigned)ns;                     // :60
+        for (int j = 0; j < 4; ++j) o[j] = c[j] / cnt;
+    }
+}
+
 int or_max_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -74,6 +74,8 @@ def lib():
                                  ctypes.POINTER(ctypes.c_int)]
         L.or_rowmask.restype = ctypes.c_ulonglong
         L.or_make_rectangle.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.or_shade.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 6
+        L.or_shade.restype = None
         L.or_max_threads.restype = ctypes.c_int
         L.or_occlude_all.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Input), ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_void_p]
@@ -203,3 +205,17 @@ def make_rectangle(mat16, p3):
 
 def max_threads():
     return lib().or_max_threads()
+
+
+def shade(shadow, g0, g1, g2, g3, samples, cam, La, Ld, Ls):
+    """phoneShading.shf:23-61 restated in float (see or_shade); returns float4
+    per texel, shape (n, 4)."""
+    f32 = lambda a: np.ascontiguousarray(np.asarray(a, np.float32))
+    shadow, g0, g1, g2, g3 = f32(shadow).ravel(), f32(g0), f32(g1), f32(g2), f32(g3)
+    samples, cam, La, Ld, Ls = f32(samples).ravel(), f32(cam), f32(La), f32(Ld), f32(Ls)
+    n = shadow.size
+    out = np.zeros((n, 4), np.float32)
+    lib().or_shade(n, shadow.ctypes.data, g0.ctypes.data, g1.ctypes.data, g2.ctypes.data, g3.ctypes.data,
+                   samples.size // 3, samples.ctypes.data, cam.ctypes.data, La.ctypes.data, Ld.ctypes.data,
+                   Ls.ctypes.data, out.ctypes.data)
+    return out

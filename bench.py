@@ -362,6 +362,92 @@ def gpu_target_config(pkg, steps=10):
                 gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
 
+def gpu_app_pipeline(pkg, frames=30):
+    """The application around the visibility path at 1080p: device G-buffer
+    producer (20 boxes on a ground plane, the boxes-1080p receivers) ->
+    visibility (boxes-1080p occluders, 1M triangles, 32x32 light, 32x32 bins)
+    -> deferred shading; each stage timed alone and the whole frame
+    end to end.  The device G-buffer is checked bit for bit against the host
+    ray cast."""
+    fd, wl = pkg.fd4d, pkg.workloads
+    W, H = 1920, 1080
+    sc = wl.gbuffer_scene_boxes(W, H, 20, 3)
+    frame = wl.boxes_1080p(light_res=32)
+    frame.grid_log2 = 5
+    strm = fd.Stream()
+    n = W * H
+    g = [fd.DeviceBuffer(n * 16) for _ in range(4)]
+    shadow, rgba = fd.DeviceBuffer(n * 4), fd.DeviceBuffer(n * 16)
+    masks = fd.DeviceBuffer(n * fd.mask_words(32) * 4)
+    ctx = fd.Context()
+    ctx.set_backface_cull(frame.backface_cull)
+    ctx.set_scenes_esp(frame.eps)
+    ctx.set_bin_num(32)
+    verts = fd.DeviceBuffer.from_array(frame.scene.verts)
+    idx = fd.DeviceBuffer.from_array(frame.scene.idx)
+    ctx.add_model(verts, frame.scene.nverts, 16, idx, frame.scene.ntris)
+    ctx.set_receivers(g[0], g[1], W, H)
+    ctx.set_outputs(shadow, masks)
+    ctx.set_area_light(frame.light)
+    L = frame.light
+    view, up = np.asarray(L.view_dir, np.float64), np.asarray(L.up_dir, np.float64)
+    right = np.cross(view / np.linalg.norm(view), up)
+    right /= np.linalg.norm(right)
+    top = np.cross(right, view / np.linalg.norm(view))
+    samples = [np.asarray(L.position) - right * L.right_radius - top * L.top_radius]   # Render.cpp:451-466
+    light_terms = (sc.eye, (0.2, 0.2, 0.2), (0.9, 0.85, 0.8), (0.5, 0.5, 0.5))
+
+    def produce():
+        fd.gbuffer_boxes(W, H, sc.boxes, sc.materials, sc.eye, sc.at, sc.up, sc.fovy_deg, sc.ground_half,
+                         g[0], g[1], g[2], g[3], stream=strm)
+
+    def shade():
+        fd.shade(W, H, samples, *light_terms, shadow, g[0], g[1], g[2], g[3], rgba, stream=strm)
+
+    def timed(fn, reps):
+        fn()
+        strm.sync()
+        t = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            fn()
+            strm.sync()
+            t.append((time.perf_counter() - a) * 1e3)
+        return t
+
+    produce()
+    strm.sync()
+    host = wl.gbuffer_boxes_host(sc)
+    exact = all(np.array_equal(g[k].download(np.float32).reshape(H, W, 4).view(np.uint32), host[k].view(np.uint32))
+                for k in range(4))
+    t_g = timed(produce, 20)
+    t_v = timed(lambda: ctx.launch(strm.handle), 10)
+    t_s = timed(shade, 20)
+
+    def whole():
+        produce()
+        rc, _ = ctx.launch(strm.handle)
+        assert rc == 0
+        shade()
+    t_all = timed(whole, frames)
+    ctx.close()
+    strm.close()
+    gb_bytes = 64 * n                      # four float4 targets written
+    sh_bytes = (4 * 16 + 4 + 16) * n       # four targets + shadow read, one float4 written
+    gms, sms = pctl(t_g, 50), pctl(t_s, 50)
+    return dict(workload="1920x1080 boxes G-buffer (20 boxes + ground, device ray cast) -> visibility "
+                "(boxes-1080p occluders, 1M triangles, 32x32 light, 32x32 bins) -> deferred shading (1 bound sample)",
+                gbuffer_ms=round(gms, 3), visibility_ms=round(pctl(t_v, 50), 3), shade_ms=round(sms, 3),
+                frame_p50_ms=round(pctl(t_all, 50), 3), frame_p99_ms=round(pctl(t_all, 99), 3), frames=frames,
+                gbuffer_bit_exact_vs_host=bool(exact),
+                gbuffer_gigarays_per_s=round(n / (gms * 1e-3) / 1e9, 3),
+                gbuffer_hbm=dict(bytes=gb_bytes, gbs=round(gb_bytes / (gms * 1e-3) / 1e9, 1),
+                                 frac=round(gb_bytes / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+                shade_hbm=dict(bytes=sh_bytes, gbs=round(sh_bytes / (sms * 1e-3) / 1e9, 1),
+                               frac=round(sh_bytes / (sms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+                note="stage times include the host round trip of one call (launch + stream sync)")
+
+
 def cpu_baseline(frame, gpu, seconds):
     """The oracle (CPU restatement of the reference) on a receiver sample,
     all host cores; full-frame time = fixed stages + occlusion scaled."""
@@ -471,6 +557,8 @@ def main():
         out["served"] = gpu_served(pkg)
         log("[bench] latency target configuration")
         out["target_1080p_32x32_1M"] = gpu_target_config(pkg, steps=100)
+        log("[bench] application pipeline (G-buffer -> visibility -> shading)")
+        out["app_pipeline"] = gpu_app_pipeline(pkg)
     if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)

@@ -266,6 +266,16 @@ __global__ void k_move_model(float* vbo, int vstride, int nv, int nthreads, M16 
 
 }  // namespace
 
+namespace fdh {
+hipError_t move_vertices(float* verts, int vstride, int nverts, const float m[16], hipStream_t st) {
+    M16 mm;
+    std::memcpy(mm.m, m, sizeof(mm.m));
+    const int grid = std::min(65536, std::max(nverts, 1));
+    hipLaunchKernelGGL(k_move_model, dim3(grid), dim3(256), 0, st, verts, vstride, nverts, grid * 256, mm);
+    return hipGetLastError();
+}
+}  // namespace fdh
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -451,12 +461,7 @@ int fd_move_model(fd_ctx* c, const float m[16], void* stream) {
     if (!c || !m) return FD_ERR_ARG;
     if (c->models.empty()) return FD_ERR_STATE;
     const Model& md = c->models[0];
-    M16 mm;
-    std::memcpy(mm.m, m, sizeof(mm.m));
-    const int grid = std::min(65536, std::max(md.nverts, 1));
-    hipLaunchKernelGGL(k_move_model, dim3(grid), dim3(256), 0, (hipStream_t)stream, (float*)md.verts, md.vstride,
-                       md.nverts, grid * 256, mm);
-    hipError_t e = hipGetLastError();
+    hipError_t e = fdh::move_vertices((float*)md.verts, md.vstride, md.nverts, m, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     return hip_status(e);
 }

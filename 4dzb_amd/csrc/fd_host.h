@@ -32,6 +32,9 @@ struct Model {
 };
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+// In-place vertex transform of a resident model (FD::moveModel,
+// FDc/moveModel.inl:6-30), enqueued on st.
+hipError_t move_vertices(float* verts, int vstride, int nverts, const float m[16], hipStream_t st);
 int hip_status(hipError_t e);
 // FD::FDsetAreaLight (4DZB/FDZB_Soft_Shadow/api.cpp:303-400)
 void light_setup(const fd_area_light& d, FdLight& L);

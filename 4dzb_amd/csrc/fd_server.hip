@@ -9,6 +9,12 @@
 // streams run concurrently; completed batches release their workspace, which
 // admits the next requests.  Each result is bit-identical to running the
 // request alone through fd_launch (same arithmetic, per-request workspace).
+//
+// Scene moves (FD::moveModel on a resident scene) travel in the same
+// arrival-ordered queue: a move waits until every running batch that holds
+// a request of its scene has finished, transforms the scene's vertices in
+// place, and only then are later requests admitted, so each request sees the
+// geometry of its submission time.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <atomic>
@@ -75,6 +81,9 @@ private:
 
 struct Req {
     long long id;
+    int kind = 0;          // 0: shadow request, 1: scene move
+    int moveScene = -1;
+    float m[16];
     fd_request r;
     FdArgs args;
     size_t bytes = 0, off = 0;
@@ -111,13 +120,41 @@ struct fd_server {
     std::chrono::steady_clock::time_point t0;
     std::atomic<int> error{0};
     int device = 0;
+    hipStream_t moveStream = nullptr;
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     void run();
     bool launch(Slot& s, std::vector<Req>& batch);
+    bool try_move();
 };
+
+// The scene move at the head of the queue, once no running batch holds a
+// request of its scene.  Returns true when it ran.
+bool fd_server::try_move() {
+    const Req& q = pending.front();
+    for (const Slot& sl : slots)
+        if (sl.busy)
+            for (const Req& r : sl.reqs)
+                if (r.r.scene == q.moveScene) return false;
+    const Model& md = scenes[q.moveScene];
+    hipError_t e = move_vertices((float*)md.verts, md.vstride, md.nverts, q.m, moveStream);
+    if (e == hipSuccess) e = hipStreamSynchronize(moveStream);
+    if (e != hipSuccess) { error = FD_ERR_HIP; stop = true; cv.notify_all(); return false; }
+    fd_completion c{};
+    c.id = q.id;
+    c.user_tag = q.r.user_tag;
+    c.status = FD_OK;
+    c.t_submit_ms = q.tSubmit;
+    c.t_start_ms = c.t_done_ms = now();
+    pending.pop_front();
+    std::lock_guard<std::mutex> lk(mu);
+    done.push_back(c);
+    ++completed;
+    cv.notify_all();
+    return true;
+}
 
 bool fd_server::launch(Slot& s, std::vector<Req>& batch) {
     std::vector<FdArgs> args;
@@ -178,11 +215,13 @@ void fd_server::run() {
             completed += (long long)out.size();
             cv.notify_all();
         }
-        // admit and launch, in arrival order
+        // a scene move at the head of the queue runs once its scene is idle
+        if (!pending.empty() && pending.front().kind == 1 && try_move()) progress = true;
+        // admit and launch, in arrival order (a move ends the batch)
         for (Slot& s : slots) {
-            if (s.busy || pending.empty()) continue;
+            if (s.busy || pending.empty() || pending.front().kind == 1) continue;
             std::vector<Req> batch;
-            while (!pending.empty() && (int)batch.size() < cfg.max_batch) {
+            while (!pending.empty() && pending.front().kind == 0 && (int)batch.size() < cfg.max_batch) {
                 Req& q = pending.front();
                 size_t off;
                 if (!pool.alloc(q.bytes, off)) break;   // wait for memory: keep arrival order
@@ -220,6 +259,7 @@ int fd_server_create(const fd_server_config* cfg, fd_server** out) {
     if (e != hipSuccess) { delete s; return hip_status(e); }
     s->pool.init(cfg->hbm_budget_bytes);
     s->slots.resize(s->cfg.streams);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->moveStream, hipStreamNonBlocking);
     for (Slot& sl : s->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
@@ -301,6 +341,26 @@ int fd_server_drain(fd_server* s, int timeout_ms) {
     return ok ? FD_OK : FD_ERR_STATE;
 }
 
+long long fd_server_move_scene(fd_server* s, int scene, const float m[16], unsigned long long user_tag) {
+    if (!s || !m) return FD_ERR_ARG;
+    Req q;
+    q.kind = 1;
+    std::memcpy(q.m, m, sizeof(q.m));
+    q.r = fd_request{};
+    q.r.user_tag = user_tag;
+    q.r.scene = -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (scene < 0 || scene >= (int)s->scenes.size()) return FD_ERR_ARG;
+    if (s->error.load()) return s->error.load();
+    q.moveScene = scene;
+    q.id = s->nextId++;
+    q.tSubmit = s->now();
+    const long long id = q.id;
+    s->incoming.push_back(std::move(q));
+    ++s->submitted;
+    return id;
+}
+
 double fd_server_now_ms(fd_server* s) { return s ? s->now() : 0.0; }
 
 int fd_server_destroy(fd_server* s) {
@@ -315,6 +375,7 @@ int fd_server_destroy(fd_server* s) {
         if (sl.staging) (void)hipHostFree(sl.staging);
         if (sl.states) (void)hipHostFree(sl.states);
     }
+    if (s->moveStream) (void)hipStreamDestroy(s->moveStream);
     s->poolMem.release();
     delete s;
     return FD_OK;

@@ -152,6 +152,7 @@ def lib():
             "fd_server_add_scene": ([vp, vp, sz, ip, vp, ip], ip),
             "fd_server_submit": ([vp, ctypes.c_void_p], ctypes.c_longlong),
             "fd_server_poll": ([vp, ctypes.c_void_p, ip], ip),
+            "fd_server_move_scene": ([vp, ip, ctypes.POINTER(ctypes.c_float), ctypes.c_ulonglong], ctypes.c_longlong),
             "fd_server_drain": ([vp, ip], ip),
             "fd_server_now_ms": ([vp], ctypes.c_double),
             "fd_server_destroy": ([vp], ip),
@@ -504,6 +505,14 @@ class Server:
         rid = lib().fd_server_submit(self.h, ctypes.cast(ctypes.pointer(r), ctypes.c_void_p))
         if rid < 0:
             raise FdError(f"fd_server_submit -> {rid}")
+        return rid
+
+    def move_scene(self, scene, m16, tag=0):
+        """fd_server_move_scene: m16 column-major 4x4, ordered with the requests."""
+        m = (ctypes.c_float * 16)(*[float(x) for x in m16])
+        rid = lib().fd_server_move_scene(self.h, scene, m, tag)
+        if rid < 0:
+            raise FdError(f"fd_server_move_scene -> {rid}")
         return rid
 
     def poll(self, max_n=256):

@@ -238,6 +238,12 @@ int fd_server_add_scene(fd_server* srv, const float* d_vertices, size_t vertex_s
                         const int* d_indices, int num_triangles);
 /* returns the request id (>= 0); FD_ERR_CAPACITY if it can never fit */
 long long fd_server_submit(fd_server* srv, const fd_request* req);
+/* FD::moveModel (FDZB_Soft_Shadow/api.cpp moveModel, FDc/moveModel.inl:6-30)
+ * for a resident scene, ordered with the requests: requests submitted before
+ * the move see the old vertices, requests submitted after it the transformed
+ * ones (m: column-major 4x4, applied in place to the scene's vertex buffer).
+ * The move completes like a request (batch_size 0); returns its id. */
+long long fd_server_move_scene(fd_server* srv, int scene, const float m[16], unsigned long long user_tag);
 /* non-blocking; copies up to max completions, returns how many */
 int fd_server_poll(fd_server* srv, fd_completion* out, int max);
 /* waits until every submitted request completed (FD_ERR_STATE on timeout) */

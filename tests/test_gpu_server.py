@@ -48,3 +48,66 @@ def test_server_results_equal_alone(fd, wl):
         assert np.array_equal(j["shadow"].download(np.float32).view(np.uint32), sh0.view(np.uint32)), k
         assert np.array_equal(j["masks"].download(np.uint32), mk0), k
     srv.close()
+
+
+def _rot_z_translate(deg, t):
+    c, s = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[0, 1], m[1, 0], m[1, 1] = c, -s, s, c
+    m[:3, 3] = t
+    return m.T.ravel()   # column-major
+
+
+def test_server_scene_moves_are_ordered(fd, wl):
+    """FD::moveModel on a resident scene through the server: requests
+    submitted before a move see the old vertices, later ones the moved
+    vertices (cumulative, in place); other scenes are unaffected."""
+    scenes = [wl.spheres(60, 11), wl.spheres(90, 12)]
+    srv = fd.Server(budget_bytes=1 << 30, streams=2, max_batch=2, eps=1e-3)
+    keep = []
+    for sc in scenes:
+        v, i = fd.DeviceBuffer.from_array(sc.verts), fd.DeviceBuffer.from_array(sc.idx)
+        keep += [v, i]
+        srv.add_scene(v, sc.nverts, 16, i, sc.ntris)
+    rx = wl.receivers_heightfield(160, 120, 8.0, 6.0)
+    light = wl.LightDesc(1.0, 1.0, (0.2, -0.3, 5.5), res_w=32, res_h=32)
+    pos, nrm = fd.DeviceBuffer.from_array(rx.pos), fd.DeviceBuffer.from_array(rx.nrm)
+    m1, m2 = _rot_z_translate(10.0, (0.3, -0.2, 0.1)), _rot_z_translate(-25.0, (-0.5, 0.4, 0.0))
+    plan = ["A", "A", "C", "M1", "B", "C", "B", "M2", "D", "C"]
+    outs = []
+    for k, step in enumerate(plan):
+        if step.startswith("M"):
+            srv.move_scene(0, m1 if step == "M1" else m2, tag=1000 + k)
+            outs.append(None)
+            continue
+        sh, mk = fd.DeviceBuffer(160 * 120 * 4), fd.DeviceBuffer(160 * 120 * fd.mask_words(32) * 4)
+        srv.submit(1 if step == "C" else 0, pos, nrm, 160, 120, light, sh, mk, tag=k)
+        outs.append((sh, mk))
+    srv.drain(60000)
+    done = {c.user_tag: c for c in srv.poll()}
+    assert sorted(done) == sorted([k for k, s in enumerate(plan) if not s.startswith("M")] +
+                                  [1000 + k for k, s in enumerate(plan) if s.startswith("M")])
+    assert all(c.status == 0 for c in done.values())
+    assert done[1003].batch_size == 0 and done[1007].batch_size == 0
+    # expected: scene 0 alone at each geometry version (the same device transform)
+    exp = {}
+    run = fd.FrameRunner(wl.Frame("alone", scenes[0], rx, light, eps=1e-3, backface_cull=1))
+    for version, m in (("A", None), ("B", m1), ("D", m2)):
+        if m is not None:
+            run.ctx.move_model(m)
+        rc, _ = run.run()
+        assert rc == 0
+        exp[version] = tuple(x.copy() for x in run.outputs())
+    run.close()
+    runc = fd.FrameRunner(wl.Frame("alone", scenes[1], rx, light, eps=1e-3, backface_cull=1))
+    rc, _ = runc.run()
+    exp["C"] = tuple(x.copy() for x in runc.outputs())
+    runc.close()
+    assert not np.array_equal(exp["A"][1], exp["B"][1]) and not np.array_equal(exp["B"][1], exp["D"][1])
+    for k, step in enumerate(plan):
+        if step.startswith("M"):
+            continue
+        sh, mk = outs[k]
+        assert np.array_equal(sh.download(np.float32).view(np.uint32), exp[step][0].view(np.uint32)), (k, step)
+        assert np.array_equal(mk.download(np.uint32), exp[step][1]), (k, step)
+    srv.close()

@@ -362,6 +362,43 @@ def gpu_target_config(pkg, steps=10):
                 gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
 
+def gpu_shard_tile(pkg, steps=10):
+    """One GPU's share of a frame sharded by screen tiles over 8 GPUs: a
+    480x540 tile (259,200 receivers, 1/8 of 1920x1080) of the boxes-1080p
+    G-buffer with its 64x64 light against all 1M triangles, autotuned layout.
+    (A real shard would take the light plane from an all-reduce over the
+    tiles; here the tile computes its own.)"""
+    fd, wl = pkg.fd4d, pkg.workloads
+    frame = wl.boxes_1080p()
+    rx = frame.rx
+    frame.rx = wl.Receivers(np.ascontiguousarray(rx.pos[540:1080, 720:1200]),
+                            np.ascontiguousarray(rx.nrm[540:1080, 720:1200]))
+    strm = fd.Stream()
+    run = fd.FrameRunner(frame, stream=strm)
+    item = dict(pos=run.pos, nrm=run.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=run.shadow,
+                max_pairs=64 << 20)
+    cands = pkg.autotune.CANDIDATES
+    best, _ = run.ctx.autotune(item, cands, 3)
+    bins, tile = cands[best] if best >= 0 else (16, 8)
+    run.close()
+    frame.grid_log2 = bins.bit_length() - 1
+    frame.tile_width = tile
+    run = fd.FrameRunner(frame, stream=strm)
+    run.run()
+    lat = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        rc, st = run.run()
+        lat.append((time.perf_counter() - a) * 1e3)
+    run.close()
+    strm.close()
+    n = frame.rx.w * frame.rx.h
+    R = frame.light.res_w
+    return dict(workload="480x540 tile (1/8 of boxes-1080p's 1920x1080) with the 64x64 light, 1M triangles",
+                layout=dict(bins=bins, tile=tile), p50_ms=round(pctl(lat, 50), 2), exit=st.exit_code,
+                gbps=round(n * R * R / (pctl(lat, 50) * 1e-3) / 1e9, 3))
+
+
 def gpu_app_pipeline(pkg, frames=30):
     """The application around the visibility path at 1080p: device G-buffer
     producer (20 boxes on a ground plane, the boxes-1080p receivers) ->
@@ -559,6 +596,8 @@ def main():
         out["target_1080p_32x32_1M"] = gpu_target_config(pkg, steps=100)
         log("[bench] application pipeline (G-buffer -> visibility -> shading)")
         out["app_pipeline"] = gpu_app_pipeline(pkg)
+        log("[bench] one GPU's tile of an 8-way screen split")
+        out["shard_tile_1_of_8"] = gpu_shard_tile(pkg)
     if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)

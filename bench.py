@@ -535,6 +535,8 @@ def main():
     ap.add_argument("--light-res", type=int, default=0, help="override the workload's light grid side")
     ap.add_argument("--bins", type=int, default=0, help="bins per light-plane side (default: the workload's, 16)")
     ap.add_argument("--tile", type=int, default=0, help="receiver tile width (default 8)")
+    ap.add_argument("--leg", choices=["batched", "served", "target", "app_pipeline", "shard_tile"], default=None,
+                    help="run only this leg and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (instances shard across GPUs "
@@ -545,6 +547,13 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     wl = pkg.workloads
+    if args.leg:
+        legs = {"batched": lambda: gpu_batched(pkg), "served": lambda: gpu_served(pkg),
+                "target": lambda: gpu_target_config(pkg, steps=args.steps),
+                "app_pipeline": lambda: gpu_app_pipeline(pkg, frames=args.steps),
+                "shard_tile": lambda: gpu_shard_tile(pkg, steps=args.steps)}
+        print(json.dumps({args.leg: legs[args.leg]()}), flush=True)
+        return
     log(f"[bench] generating {args.workload}")
     make = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
         args.workload]

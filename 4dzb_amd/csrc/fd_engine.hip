@@ -188,12 +188,13 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     a.triOff = (uint32_t*)take(T * sizeof(uint32_t));
     a.pairKT = (uint32_t*)take((size_t)a.pairCap * sizeof(uint32_t));
     a.sortedTri = (int*)take((size_t)a.pairCap * sizeof(int));
-    int* tb = (int*)take((size_t)(5 * (nb + 2)) * sizeof(int));
+    int* tb = (int*)take((size_t)(6 * (nb + 2)) * sizeof(int));
     a.rxStart = tb;
     a.rxEnd = tb ? tb + (nb + 2) : nullptr;
     a.triStart = tb ? tb + 2 * (nb + 2) : nullptr;
     a.triEnd = tb ? tb + 3 * (nb + 2) : nullptr;
     a.chunkStart = tb ? tb + 4 * (nb + 2) : nullptr;
+    a.binAtRank = tb ? tb + 5 * (nb + 2) : nullptr;
     const long np = std::max(std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)),
                              fdk::scan_partials_needed(T)) + 1;
     a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));

@@ -53,7 +53,8 @@ struct FdArgs {
     int* rxEnd;           // nb + 1
     int* triStart;        // nb
     int* triEnd;          // nb
-    int* chunkStart;      // nb + 2
+    int* chunkStart;      // nb + 2: work-item prefix over the bins in occlusion order (by rank)
+    int* binAtRank;       // nb + 1: occlusion order of the bins, longest triangle list first
     uint64_t* scanPart;   // scan partial sums
     uint32_t* part;       // per-block partials: 8 per k_reduce block (value-major), then 1 per pair-count block
     unsigned long long* opCount;  // optional occlusion counters (FD_NOPS), or null

@@ -891,29 +891,83 @@ __device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk)
 // --------------------------------------------------------------------------
 // Per-bin tables (FindStart_kernel, binRaster.inl:5-59) and the occlusion
 // work list: one work item per 64 sorted receivers of a bin (the invalid key
-// included, so every receiver's output is written exactly once).
+// included, so every receiver's output is written exactly once).  Work items
+// are issued bin by bin, longest triangle list first (by power-of-two class
+// of the list length; the invalid key, with no list, last), so that the
+// launch does not end on a few long-running waves.  The order only schedules
+// work: every receiver's result is the same in any order.
 // --------------------------------------------------------------------------
 __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
     __shared__ int wsum[16];
+    __shared__ int classCount[33], classBase[33];
     FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb;
-    int carry = 0;
+    if (threadIdx.x < 33) classCount[threadIdx.x] = 0;
+    __syncthreads();
+    // bin tables and the cost class of every bin (0 = longest lists)
+    for (int k = threadIdx.x; k <= nb; k += blockDim.x) {
+        const int rs = (int)a.histR[(size_t)k * a.nblkR];
+        const int re = (k < nb) ? (int)a.histR[(size_t)(k + 1) * a.nblkR] : a.WH;
+        a.rxStart[k] = rs;
+        a.rxEnd[k] = re;
+        int cls = 32;
+        if (k < nb) {
+            const int ts = (int)a.histT[(size_t)k * a.nblkT];
+            const int te = (k < nb - 1) ? (int)a.histT[(size_t)(k + 1) * a.nblkT] : (int)s->pairs;
+            a.triStart[k] = ts;
+            a.triEnd[k] = te;
+            cls = __clz(te - ts + 1);   // 31 - floor(log2(len + 1)): longer lists, smaller class
+        }
+        atomicAdd(&classCount[cls], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int i = 0; i < 33; ++i) { classBase[i] = c; c += classCount[i]; }
+    }
+    __syncthreads();
+    // stable within a class (ascending bin index, so that neighbouring bins,
+    // which share triangles, stay close in the issue order): ranks from wave
+    // ballots, then a prefix over the waves of each 1024-bin slice
+    __shared__ int waveCount[16][33];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k0 = 0; k0 <= nb; k0 += 1024) {
         const int k = k0 + threadIdx.x;
-        int nch = 0;
-        if (k <= nb) {
-            const int rs = (int)a.histR[(size_t)k * a.nblkR];
-            const int re = (k < nb) ? (int)a.histR[(size_t)(k + 1) * a.nblkR] : a.WH;
-            a.rxStart[k] = rs;
-            a.rxEnd[k] = re;
-            nch = (re - rs + 63) >> 6;
-            if (k < nb) {
-                const int ts = (int)a.histT[(size_t)k * a.nblkT];
-                const int te = (k < nb - 1) ? (int)a.histT[(size_t)(k + 1) * a.nblkT] : (int)s->pairs;
-                a.triStart[k] = ts;
-                a.triEnd[k] = te;
+        const bool in = k <= nb;
+        const int cls = !in ? 0 : (k < nb) ? __clz(a.triEnd[k] - a.triStart[k] + 1) : 32;
+        for (int i = lane; i < 33; i += 64) waveCount[w][i] = 0;
+        int rankInWave = 0;
+        for (uint64_t pending = __ballot(in); pending;) {
+            const int lead = __ffsll((unsigned long long)pending) - 1;
+            const int c = __shfl(cls, lead, 64);
+            const uint64_t m = __ballot(in && cls == c);
+            if (in && cls == c) rankInWave = mbcnt(m);
+            if (lane == lead) waveCount[w][c] = __popcll(m);
+            pending &= ~m;
+        }
+        __syncthreads();
+        if (threadIdx.x < 33) {   // per class: exclusive prefix over the waves, then advance the class base
+            int run = classBase[threadIdx.x];
+            for (int j = 0; j < 16; ++j) {
+                const int c = waveCount[j][threadIdx.x];
+                waveCount[j][threadIdx.x] = run;
+                run += c;
             }
+            classBase[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (in) a.binAtRank[waveCount[w][cls] + rankInWave] = k;
+        __syncthreads();
+    }
+    // work-item prefix in rank order
+    int carry = 0;
+    for (int k0 = 0; k0 <= nb; k0 += 1024) {
+        const int r = k0 + threadIdx.x;
+        int nch = 0;
+        if (r <= nb) {
+            const int k = a.binAtRank[r];
+            nch = (a.rxEnd[k] - a.rxStart[k] + 63) >> 6;
         }
         // block exclusive scan of nch
         int x = nch;
@@ -929,7 +983,7 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
             if (i < w) off += wsum[i];
             all += wsum[i];
         }
-        if (k <= nb) a.chunkStart[k] = off + x - nch;
+        if (r <= nb) a.chunkStart[r] = off + x - nch;
         carry += all;
         __syncthreads();
     }
@@ -1259,8 +1313,9 @@ __device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, c
     }
 }
 
-// Work item -> (bin, first sorted position): binary search over the chunk prefix.
-__device__ __forceinline__ int chunk_bin(const FdArgs& a, int c, int nb) {
+// Work item -> rank of its bin in the occlusion order: binary search over
+// the work-item prefix (the bin is binAtRank[rank]).
+__device__ __forceinline__ int chunk_rank(const FdArgs& a, int c, int nb) {
     int lo = 0, hi = nb + 1;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -1311,7 +1366,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     }
     if (c >= s->totalChunks) return;
     const int nb = s->nb;
-    const int bin = chunk_bin(a, c, nb);
+    const int rank = chunk_rank(a, c, nb);
+    const int bin = uniform(a.binAtRank[rank]);
     if (bin == nb && seg != 0) return;
     // this work item's stretch of the bin's triangle list (bins 0..nb-1)
     int ts = 0, te = 0;
@@ -1325,7 +1381,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             if (ts >= te) return;   // nothing to add; the finishing pass counts the merged masks
         }
     }
-    const int s0 = a.rxStart[bin] + (c - a.chunkStart[bin]) * 64;
+    const int s0 = a.rxStart[bin] + (c - a.chunkStart[rank]) * 64;
     const int n = min(64, a.rxEnd[bin] - s0);
     const bool act = lane < n;
     int id = 0;
@@ -1613,9 +1669,10 @@ __device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK || c >= s->totalChunks) return;
     const int nb = s->nb;
-    const int bin = chunk_bin(a, c, nb);
+    const int rank = chunk_rank(a, c, nb);
+    const int bin = a.binAtRank[rank];
     if (bin == nb) return;
-    const int s0 = a.rxStart[bin] + (c - a.chunkStart[bin]) * 64;
+    const int s0 = a.rxStart[bin] + (c - a.chunkStart[rank]) * 64;
     const int n = min(64, a.rxEnd[bin] - s0);
     const int lane = lane_id();
     if (lane >= n) return;

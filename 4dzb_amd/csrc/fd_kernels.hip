@@ -852,22 +852,99 @@ __device__ __forceinline__ void tri_write_body(const FdArgs& a, const int blk) {
 // (first use: the block's scanned per-bin offset).  Result: bin-major,
 // ascending triangle id within a bin.
 __device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk) {
-    extern __shared__ int cnt[];  // nb, -1 = not loaded yet
+    extern __shared__ int cnt[];  // nb running offsets, starting at the block's scanned histogram
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb;
     int kbits = 1;
     while ((1 << kbits) < nb) ++kbits;
-    for (int k = threadIdx.x; k < nb; k += 64) cnt[k] = -1;
+    for (int k = threadIdx.x; k < nb; k += 64) cnt[k] = (int)a.histT[(size_t)k * a.nblkT + blk];
     __syncthreads();
     const int t0 = blk * FD_TRB;
     const int t1 = min(t0 + FD_TRB, a.ntris);
     const unsigned long long p0 = a.triOff[t0];
     const unsigned long long p1 = t1 < a.ntris ? (unsigned long long)a.triOff[t1] : s->pairs;
     const unsigned long long cap = a.pairCap;
+    const unsigned long long e = p1 < cap ? p1 : cap;
+    // the next chunk of pairs is loaded while the current one is ranked
+    uint32_t vNext = (p0 + lane_id() < e) ? a.pairKT[p0 + lane_id()] : 0u;
     for (unsigned long long c0 = p0; c0 < p1; c0 += 64) {
         const unsigned long long p = c0 + lane_id();
-        const bool act = p < p1 && p < cap;
+        const bool act = p < e;
+        const uint32_t v = vNext;
+        const unsigned long long pn = p + 64;
+        vNext = (pn < e) ? a.pairKT[pn] : 0u;
+        const int key = (int)(v & 0xffffu);
+        uint64_t eq = __ballot(act);
+        for (int b = 0; b < kbits; ++b) {
+            const bool bit = (key >> b) & 1;
+            const uint64_t bal = __ballot(act && bit);
+            eq &= bit ? bal : ~bal;
+        }
+        if (act) {
+            const int rank = mbcnt(eq);
+            const int gcount = __popcll(eq);
+            const int b0 = cnt[key];
+            const unsigned long long dst = (unsigned long long)(uint32_t)b0 + (unsigned)rank;
+            if (rank == gcount - 1) cnt[key] = b0 + gcount;
+            if (dst < cap) a.sortedTri[dst] = t0 + (int)(v >> 16);
+        }
+    }
+}
+
+// The same stable scatter with the block's 16 waves working at once: wave w
+// owns triangles t0 + 64w .. t0 + 64w + 63 and with them a contiguous slice
+// of the triangle-major pair list.  A triangle has at most one pair per bin,
+// so a wave holds at most 64 pairs of a bin and the 16 per-wave counts of a
+// bin fit 16-bit halves of LDS words (two waves per word, counted with 32-bit
+// LDS adds).  After a barrier every bin's counts become exclusive prefixes
+// over the waves (at most 1024, still 16 bits), and each wave ranks its
+// slice as the single-wave scatter does, from base + prefix.  The destination
+// of every pair is the single-wave scatter's.  LDS: FD_SCATTER_WIDE_LDS bytes
+// per bin.
+__device__ __forceinline__ int wide_half(int w, int k, int nb) { return (((w >> 1) * nb + k) << 1) | (w & 1); }
+
+__device__ __forceinline__ void tri_scatter_wide_body(const FdArgs& a, const int blk) {
+    extern __shared__ uint32_t wlds[];
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK) return;
+    const int nb = s->nb;
+    uint32_t* cnt32 = wlds;                                   // 8 * nb words
+    uint16_t* cnt16 = (uint16_t*)wlds;
+    uint32_t* base = wlds + 8 * nb;                           // nb block offsets
+    int kbits = 1;
+    while ((1 << kbits) < nb) ++kbits;
+    for (int k = threadIdx.x; k < 8 * nb; k += FD_TRB) cnt32[k] = 0u;
+    for (int k = threadIdx.x; k < nb; k += FD_TRB) base[k] = a.histT[(size_t)k * a.nblkT + blk];
+    __syncthreads();
+    const int w = (int)(threadIdx.x >> 6);
+    const int ta = blk * FD_TRB + w * 64;
+    const int tb = min(ta + 64, a.ntris);
+    unsigned long long p0 = 0, p1 = 0;
+    if (ta < a.ntris) {
+        p0 = a.triOff[ta];
+        p1 = tb < a.ntris ? (unsigned long long)a.triOff[tb] : s->pairs;
+    }
+    const unsigned long long cap = a.pairCap;
+    const unsigned long long e = p1 < cap ? p1 : cap;
+    const uint32_t inc = (w & 1) ? 0x10000u : 1u;
+    for (unsigned long long p = p0 + lane_id(); p < e; p += 64)
+        atomicAdd(&cnt32[(w >> 1) * nb + (int)(a.pairKT[p] & 0xffffu)], inc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += FD_TRB) {
+        uint32_t run = 0;
+        for (int v = 0; v < 16; ++v) {
+            const int h = wide_half(v, k, nb);
+            const uint32_t c = cnt16[h];
+            cnt16[h] = (uint16_t)run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const int t0 = blk * FD_TRB;
+    for (unsigned long long c0 = p0; c0 < p1; c0 += 64) {
+        const unsigned long long p = c0 + lane_id();
+        const bool act = p < e;
         const uint32_t v = act ? a.pairKT[p] : 0u;
         const int key = (int)(v & 0xffffu);
         uint64_t eq = __ballot(act);
@@ -879,10 +956,10 @@ __device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk)
         if (act) {
             const int rank = mbcnt(eq);
             const int gcount = __popcll(eq);
-            int b0 = cnt[key];
-            if (b0 < 0) b0 = (int)a.histT[(size_t)key * a.nblkT + blk];
-            const unsigned long long dst = (unsigned long long)(uint32_t)b0 + (unsigned)rank;
-            if (rank == gcount - 1) cnt[key] = b0 + gcount;
+            const int h = wide_half(w, key, nb);
+            const uint32_t b0 = cnt16[h];
+            const unsigned long long dst = (unsigned long long)base[key] + b0 + (unsigned)rank;
+            if (rank == gcount - 1) cnt16[h] = (uint16_t)(b0 + gcount);
             if (dst < cap) a.sortedTri[dst] = t0 + (int)(v >> 16);
         }
     }
@@ -1700,6 +1777,7 @@ __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) { rx_scatter_body(a
 __global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) { tri_count_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_write(FdArgs a) { tri_write_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) { tri_scatter_body(a, blockIdx.x); }
+__global__ __launch_bounds__(FD_TRB) void k_tri_scatter_wide(FdArgs a) { tri_scatter_wide_body(a, blockIdx.x); }
 __global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_FIN_THREADS) void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
@@ -1763,6 +1841,7 @@ FD_BATCHED(kb_rx_scatter, 64, rx_scatter_body)
 FD_BATCHED(kb_tri_count, FD_TRB, tri_count_body)
 FD_BATCHED(kb_tri_write, FD_TRB, tri_write_body)
 FD_BATCHED(kb_tri_scatter, 64, tri_scatter_body)
+FD_BATCHED(kb_tri_scatter_wide, FD_TRB, tri_scatter_wide_body)
 FD_BATCHED(kb_tables, 1024, tables_body)
 FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
 #undef FD_BATCHED
@@ -1910,7 +1989,10 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 2);
         kb_scan_apply<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
         kb_tri_write<<<L(FD_MAP_TRI), FD_TRB, binLds, st>>>(p.inst, p.map[FD_MAP_TRI], n);
-        kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        if (nb <= FD_SCATTER_WIDE_BINS)
+            kb_tri_scatter_wide<<<L(FD_MAP_TRI), FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        else
+            kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
     }
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
@@ -1982,7 +2064,10 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         e = scan_u32(a.triOff, a.ntris, a.scanPart, &s->pairs, &s->status, st);
         if (e != hipSuccess) return e;
         k_tri_write<<<a.nblkT, FD_TRB, binLds, st>>>(a);
-        k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);
+        if (nb <= FD_SCATTER_WIDE_BINS)
+            k_tri_scatter_wide<<<a.nblkT, FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(a);
+        else
+            k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);
     } else {
         k_post_pairs<<<1, FD_FIN_THREADS, 0, st>>>(a);
     }

@@ -90,6 +90,7 @@ struct fd_ctx {
     float eps = 1e-6f;
     int gridLog2 = 4;
     int tileLog2 = 3;     // 8x8 receiver tiles
+    int occVariant = FD_OCC_PER_SAMPLE;
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
     std::vector<Model> models;
@@ -216,6 +217,7 @@ int prepare(fd_ctx* c, FdArgs& a) {
         return FD_ERR_UNSUPPORTED_LIGHT;
     fill_args(a, c->models[0], c->pos, c->nrm, c->W, c->H, c->light, c->eps, c->cull, c->gridLog2, c->tileLog2, c->shadow,
               c->masks, c->pairCap);
+    a.occVariant = c->occVariant;
     FdArgs probe = a;
     const size_t bytes = layout(probe, nullptr, c->countOps);
     hipError_t e = c->arena.ensure(bytes);
@@ -311,6 +313,12 @@ int fd_destroy(fd_ctx* c) {
 int fd_set_backface_cull(fd_ctx* c, int enable) {
     if (!c) return FD_ERR_ARG;
     c->cull = enable ? 1 : 0;
+    return FD_OK;
+}
+
+int fd_set_occlusion_variant(fd_ctx* c, int variant) {
+    if (!c || (variant != FD_OCC_PER_SAMPLE && variant != FD_OCC_BIN_QUEUE)) return FD_ERR_ARG;
+    c->occVariant = variant;
     return FD_OK;
 }
 

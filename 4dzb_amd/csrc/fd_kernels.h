@@ -67,6 +67,8 @@ struct FdArgs {
     // pair passes: bin statistics staged in LDS (the launch allocates
     // FD_BIN_LDS_BYTES per bin besides the counters); set when nb <= FD_LDS_BINS
     int binLds;
+    // occlusion kernel: FD_OCC_PER_SAMPLE (the contract) or FD_OCC_BIN_QUEUE
+    int occVariant;
 };
 
 #define FD_LDS_BINS 1024

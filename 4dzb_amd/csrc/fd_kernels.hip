@@ -1401,7 +1401,166 @@ __device__ __forceinline__ int chunk_rank(const FdArgs& a, int c, int nb) {
     return uniform(lo);
 }
 
-template <typename T, bool COUNT>
+// --------------------------------------------------------------------------
+// Bin-queue occlusion variant (shadowCal_CTA_kernel, Step5_2_shadowCal.inl:
+// 408-1100, dead code in the reference).  Its arithmetic is not the
+// per-sample kernel's: edge planes from dot(cross(edge, axis), S - v), a
+// fourth plane (the triangle's own) in the row masks, other boundary tests,
+// and mask rows along y_delt (bit j of row i: upRight + j x_delt + i y_delt).
+// Not part of the per-sample bit-exact contract; selected per context.
+//
+// Work items and outputs are the per-sample kernel's (64 sorted receivers of
+// a bin, the bin's triangle list or a stretch of it).  The lanes take 64
+// consecutive triangles, two groups of 32 (the reference's warps: a group
+// applies the triangle plane when any of its triangles reaches a row and is
+// not entirely in front of the receiver, :876/:929), and the wave walks the
+// receivers one by one, their data broadcast from the owning lane.  Row
+// masks are merged with LDS atomic ORs into the receivers' rows; rows that
+// become full are flagged and skipped (as the reference's flag word, :162-164).
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T cta_shl(int k) { return (k >= 0 && k < Bits<T>::width) ? ((T)1 << k) : (T)0; }
+
+// calLightRowRange (:302-357); bits >= W are cleared by the caller
+template <typename T>
+__device__ __forceinline__ T cta_row_range(float B, float D, int W) {
+    if (B <= 0.0f && D <= 0.0f) return (T)0;
+    if (B == 0.0f || D == 0.0f) return cta_shl<T>(W) - (T)1;
+    const int k = (int)fmaxf(0.0f, fminf((float)W, ceilf(-1.0f * D / B)));   // NaN -> W
+    const T m = cta_shl<T>(k) - (T)1;
+    return B < 0.0f ? m : ~m;
+}
+
+// calLightHeightRange (:225-300), H = W = R
+template <typename T>
+__device__ __forceinline__ T cta_height_range(float A, float B, float C, int R, T fullH) {
+    const float D = fmaxf(B * (float)(R - 1) + C, C);
+    if (A <= 0.0f && D <= 0.0f) return (T)0;
+    if (A == 0.0f || D == 0.0f) return fullH;
+    const int k = (int)fmaxf(fminf(ceilf(-1.0f * D / A), (float)R), 0.0f);   // NaN -> R
+    const T m = cta_shl<T>(k) - (T)1;
+    return A < 0.0f ? m : (~m & fullH);
+}
+
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// The triangle loop of the variant for one work item: n receivers (lane s
+// holds receiver s in q), triangle list entries [ts, te).  Rows go to lds
+// (row r of receiver s at lds[r * 64 + s], high words R * 64 further).
+template <typename T>
+__device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* fullRows, int ts, int te, int n,
+                                         const FdRxRec& q, int R) {
+    const int lane = lane_id();
+    const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);   // lightWidthFullBit
+    // (T)(1 << H) - 1 with the int shift of the reference (:431): all ones of T from H = 32 on
+    const T fullH = (R >= 32) ? ~(T)0 : (((T)1 << R) - (T)1);
+    const float xd0 = a.L.xd[0], xd1 = a.L.xd[1], xd2 = a.L.xd[2];
+    const float yd0 = a.L.yd[0], yd1 = a.L.yd[1], yd2 = a.L.yd[2];
+    const float ur0 = a.L.ur[0], ur1 = a.L.ur[1], ur2 = a.L.ur[2];
+    const float eps = a.eps;
+    const uint64_t half = lane < 32 ? 0x00000000ffffffffull : 0xffffffff00000000ull;
+    for (int tb = ts; tb < te; tb += 64) {
+        const int ti = tb + lane;
+        int tid = ti < te ? a.sortedTri[ti] : -1;
+        const bool has = (unsigned)tid < (unsigned)a.ntris;
+        float v[3][3] = {}, bminx = 3.402823466e+38f, bminy = 3.402823466e+38f, bmaxx = -3.402823466e+38f,
+              bmaxy = -3.402823466e+38f, minz = 3.402823466e+38f, maxz = -3.402823466e+38f;
+        if (has) {
+            const FdTri& t = a.tri[tid];
+            for (int k = 0; k < 3; ++k) { v[0][k] = t.v0[k]; v[1][k] = t.v1[k]; v[2][k] = t.v2[k]; }
+            bminx = t.bminx; bminy = t.bminy; bmaxx = t.bmaxx; bmaxy = t.bmaxy;
+            minz = t.minz; maxz = t.maxz;
+        }
+        // receiver-independent parts (:667-674, and the cross products of :720-742):
+        // edge k runs from v_k to v_{k+1} (v0_v1, v1_v2, v2_v0), its plane is
+        // dot(cross(edge, axis), S - v_{k+1})
+        float cy[3][3], cx[3][3], cl[3][3];
+        for (int k = 0; k < 3; ++k) {
+            const int j = (k + 1) % 3;
+            const float ex = v[k][0] - v[j][0], ey = v[k][1] - v[j][1], ez = v[k][2] - v[j][2];
+            const float lx = ur0 - v[j][0], ly = ur1 - v[j][1], lz = ur2 - v[j][2];
+            cy[k][0] = ey * yd2 - ez * yd1; cy[k][1] = ez * yd0 - ex * yd2; cy[k][2] = ex * yd1 - ey * yd0;
+            cx[k][0] = ey * xd2 - ez * xd1; cx[k][1] = ez * xd0 - ex * xd2; cx[k][2] = ex * xd1 - ey * xd0;
+            cl[k][0] = ey * lz - ez * ly;   cl[k][1] = ez * lx - ex * lz;   cl[k][2] = ex * ly - ey * lx;
+        }
+        // n = cross(v1 - v2, v2 - v0); A3, B3, C3 = n . y_delt, n . x_delt, n . (upRight - v0)
+        float nn[3];
+        {
+            const float ax = v[1][0] - v[2][0], ay = v[1][1] - v[2][1], az = v[1][2] - v[2][2];
+            const float bx = v[2][0] - v[0][0], by = v[2][1] - v[0][1], bz = v[2][2] - v[0][2];
+            nn[0] = ay * bz - az * by; nn[1] = az * bx - ax * bz; nn[2] = ax * by - ay * bx;
+        }
+        const float A3o = dot3(nn[0], nn[1], nn[2], yd0, yd1, yd2);
+        const float B3o = dot3(nn[0], nn[1], nn[2], xd0, xd1, xd2);
+        const float C3o = dot3(nn[0], nn[1], nn[2], ur0 - v[0][0], ur1 - v[0][1], ur2 - v[0][2]);
+        if (!__any(has)) break;
+        for (int s = 0; s < n; ++s) {
+            const T done = fullRows[s];
+            if (done == FULL) continue;   // every row of this receiver is full
+            const float rminx = bcast(q.rminx, s), rminy = bcast(q.rminy, s);
+            const float rmaxx = bcast(q.rmaxx, s), rmaxy = bcast(q.rmaxy, s), rz = bcast(q.z, s);
+            // :686-689
+            const bool in = bminx <= rminx && bminy <= rminy && rmaxx <= bmaxx && rmaxy <= bmaxy && minz <= rz;
+            if (!__any(in)) continue;
+            const float S0 = bcast(q.q[0], s), S1 = bcast(q.q[1], s), S2 = bcast(q.q[2], s);
+            T rows = 0;
+            float A[4], B[4], C[4];
+            if (in) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int j = (k + 1) % 3;
+                    const float sx = S0 - v[j][0], sy = S1 - v[j][1], sz = S2 - v[j][2];
+                    A[k] = dot3(cy[k][0], cy[k][1], cy[k][2], sx, sy, sz);
+                    B[k] = dot3(cx[k][0], cx[k][1], cx[k][2], sx, sy, sz);
+                    C[k] = dot3(cl[k][0], cl[k][1], cl[k][2], sx, sy, sz);
+                }
+                A[3] = A3o; B[3] = B3o; C[3] = C3o;
+                const float temp = dot3(nn[0], nn[1], nn[2], S0 - v[0][0], S1 - v[0][1], S2 - v[0][2]);   // :772
+                const bool skip = fabsf(temp) < eps && point_in_tri(v[0], v[1], v[2], S0, S1, S2);   // :783
+                if (!skip) {
+                    if (temp > 0.0f) {   // :785-802
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) { A[k] = -A[k]; B[k] = -B[k]; C[k] = -C[k]; }
+                    }
+                    rows = fullH;   // :821-842
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) rows &= cta_height_range<T>(A[k], B[k], C[k], R, fullH);
+                }
+            }
+            // the group's triangle-plane decision (:876, :929), on the rows
+            // before the removal of full rows
+            const bool needDepth = rows != (T)0 && !(maxz < rz);
+            const bool depth = (__ballot(needDepth) & half) != 0ull;
+            rows &= FULL & ~done;   // rows >= H are not part of the grid
+            while (rows) {
+                const int i = Bits<T>::ctz(rows);
+                const float fi = (float)i;
+                T m = FULL;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) m &= cta_row_range<T>(B[k], A[k] * fi + C[k], R);
+                if (depth) m &= cta_row_range<T>(B[3], A[3] * fi + C[3], R);
+                m &= FULL;
+                if (m != (T)0) {
+                    T cur;
+                    if (Bits<T>::width == 32) {
+                        cur = (T)(atomicOr(lds + i * 64 + s, (uint32_t)m) | (uint32_t)m);
+                    } else {
+                        const uint64_t mm = (uint64_t)m;
+                        const uint32_t lo = atomicOr(lds + i * 64 + s, (uint32_t)mm) | (uint32_t)mm;
+                        const uint32_t hi = atomicOr(lds + (R + i) * 64 + s, (uint32_t)(mm >> 32)) | (uint32_t)(mm >> 32);
+                        cur = (T)(((uint64_t)hi << 32) | lo);
+                    }
+                    if (cur == FULL) atomicOr(fullRows + s, (T)1 << i);
+                }
+                rows &= rows - (T)1;
+            }
+        }
+    }
+}
+
+template <typename T, bool COUNT, bool CTA = false>
 __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 ids
     const FdState* s = a.st;
@@ -1452,7 +1611,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         ts = a.triStart[bin];
         te = a.triEnd[bin];
         if (S > 1) {
-            const int per = (te - ts + S - 1) / S;
+            int per = (te - ts + S - 1) / S;
+            if (CTA) per = (per + 31) & ~31;   // stretches keep the groups of 32
             ts += seg * per;
             te = min(te, ts + per);
             if (ts >= te) return;   // nothing to add; the finishing pass counts the merged masks
@@ -1488,6 +1648,25 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     for (int r = 0; r < R; ++r) {
         lds[r * 64 + lane] = 0u;
         if (wpr == 2) lds[(R + r) * 64 + lane] = 0u;
+    }
+    if (CTA) {
+        T* fullRows = (T*)(ids + 64);
+        fullRows[lane] = (T)0;
+        __syncthreads();
+        cta_tris<T>(a, lds, fullRows, ts, te, n, q, R);
+        __syncthreads();
+        if (S == 1) {
+            int cnt = 0;
+            for (int r = 0; r < R; ++r) {
+                cnt += __popc(lds[r * 64 + lane]);
+                if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
+            }
+            if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;   // :1080-1088
+            store_masks<T>(a, lds, ids, n, R, wpr);
+        } else {
+            or_masks(a, lds, ids, n, R, wpr);
+        }
+        return;
     }
     const float Lx = a.L.ur[0] - q.q[0], Ly = a.L.ur[1] - q.q[1], Lz = a.L.ur[2] - q.q[2];  // L0 - Q
     const float xd0 = a.L.xd[0], xd1 = a.L.xd[1], xd2 = a.L.xd[2];
@@ -1801,6 +1980,10 @@ template <typename T, bool COUNT>
 __global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
     occlude_body<T, COUNT>(a, xcd_block(blockIdx.x, gridDim.x));
 }
+template <typename T>
+__global__ __launch_bounds__(64) void k_occlude_cta(FdArgs a) {
+    occlude_body<T, false, true>(a, xcd_block(blockIdx.x, gridDim.x));
+}
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ d, long n,
                                                             uint64_t* __restrict__ part, const int* status) {
     scan_reduce_body(d, n, part, status, blockIdx.x);
@@ -2080,7 +2263,11 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     const int wpr = a.L.wpr;
     const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * sizeof(int);
     if (a.segs > 1) k_zero_masks<<<zero_blocks(a), 256, 0, st>>>(a);
-    if (a.opCount) {
+    if (a.occVariant == FD_OCC_BIN_QUEUE) {
+        const size_t shmCta = shm + 64 * sizeof(uint64_t);   // per-receiver full-row flags
+        if (wpr == 1) k_occlude_cta<uint32_t><<<chunks, 64, shmCta, st>>>(a);
+        else k_occlude_cta<uint64_t><<<chunks, 64, shmCta, st>>>(a);
+    } else if (a.opCount) {
         if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64, shm, st>>>(a);
         else k_occlude<uint64_t, true><<<chunks, 64, shm, st>>>(a);
     } else {

@@ -27,6 +27,8 @@ FD_ERR_NO_DEVICE = -6
 EXIT_OK, EXIT_SAMPLE_MAXZ, EXIT_TRI_MAXZ, EXIT_NO_VALID_SAMPLE = 0, 1, 2, 3
 EXIT_EMPTY_AABB, EXIT_EMPTY_LP, EXIT_NO_VALID_TRI, EXIT_NO_PAIRS, EXIT_CAPACITY = 4, 5, 6, 7, 9
 
+OCC_PER_SAMPLE, OCC_BIN_QUEUE = 0, 1   # fd_set_occlusion_variant
+
 
 class AreaLight(ctypes.Structure):
     _fields_ = [("right_radius", ctypes.c_float), ("top_radius", ctypes.c_float),
@@ -125,6 +127,7 @@ def lib():
             "fd_unregister_res": ([vp], ip),
             "fd_set_bin_num": ([vp, sz], ip),
             "fd_set_tile_width": ([vp, ip], ip),
+            "fd_set_occlusion_variant": ([vp, ip], ip),
             "fd_autotune": ([vp, ctypes.c_void_p, ctypes.c_void_p, ip, ip, ctypes.POINTER(ip)], ip),
             "fd_set_view": ([vp, ip, ip], ip),
             "fd_set_scenes_esp": ([vp, ctypes.c_float], ip),
@@ -305,6 +308,12 @@ class Context:
 
     def set_tile_width(self, tw):
         _check(lib().fd_set_tile_width(self.h, tw), f"fd_set_tile_width({tw})")
+
+    def set_occlusion_variant(self, variant):
+        """OCC_PER_SAMPLE (the reference's FDLaunch kernel, bit-exact contract)
+        or OCC_BIN_QUEUE (its unused shadowCal_CTA_kernel; other arithmetic,
+        mask rows along y_delt)."""
+        _check(lib().fd_set_occlusion_variant(self.h, int(variant)), f"fd_set_occlusion_variant({variant})")
 
     def autotune(self, inst_item, candidates, reps=3):
         """Autotuner (fd_autotune): inst_item as a Batch item (its outputs are

@@ -123,6 +123,19 @@ int fd_set_bin_num(fd_ctx* ctx, size_t bins_per_side);
  * {8, 16, 32, 64} (default 8).  Changes speed only, never a result. */
 int fd_set_tile_width(fd_ctx* ctx, int tile_width);
 
+/* Occlusion kernel of fd_launch / fd_launch_async.  FD_OCC_PER_SAMPLE is
+ * shadowCal_per_sample_kernel (FDc/Step5_2_shadowCalPerSample.inl:2-259),
+ * the kernel FD::FDLaunch runs and the bit-exact contract of this library.
+ * FD_OCC_BIN_QUEUE is the reference's unused bin-queue kernel
+ * shadowCal_CTA_kernel (FDc/Step5_2_shadowCal.inl:408-1100, with
+ * prepareBin, FDc/Step5_1_prepareBin.inl:75-139): other arithmetic (a
+ * fourth, triangle plane; other boundary tests) and mask rows along y_delt
+ * (bit j of row i is the sample upRight + j x_delt + i y_delt).  The batched
+ * and serving forms always use FD_OCC_PER_SAMPLE. */
+#define FD_OCC_PER_SAMPLE 0
+#define FD_OCC_BIN_QUEUE 1
+int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
+
 /* FD::FDsetView, FD/api.cpp:205-237. */
 int fd_set_view(fd_ctx* ctx, int width, int height);
 

@@ -209,6 +209,7 @@ struct Input {
     int gridLog2;
     const int* subset; int nsubset;   // occlusion only for these dense receivers (nullptr = all)
     int nthreads;
+    int variant;                      // occlusion: 0 per-sample kernel, 1 bin-queue (CTA) kernel
 };
 
 enum Exit {
@@ -355,6 +356,115 @@ int occludeOne(const Light& L, float eps, f3 Q, f4 sp, float spz, const int* tri
             valid ^= (1ull << r);
         }
         if (cclip) ++st.colClip;
+    }
+    int cnt = 0;
+    for (int r = 0; r < H; ++r) cnt += __builtin_popcountll(mask[r]);
+    return cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Bin-queue (CTA) occlusion variant: shadowCal_CTA_kernel,
+// FDc/Step5_2_shadowCal.inl:408-1100 (dead code in the reference, whose
+// FDLaunch runs the per-sample kernel).  Its arithmetic differs from the
+// per-sample kernel's: other plane products, a fourth (triangle) plane, other
+// boundary tests, and mask rows along y_delt (bit j of row i is the sample
+// upRight + j x_delt + i y_delt).  Pinned semantics besides the header's:
+//   * triangles are taken in groups of 32 consecutive entries of the bin's
+//     list (the reference's warps); a group applies the triangle plane to
+//     all its rows when any of its triangles has a non-empty row range and
+//     maxz >= the receiver's z (:876, :929).  The reference decides this on
+//     row ranges from which the rows already full were removed, which other
+//     warps set concurrently; here the ranges before that removal are used.
+//   * rows >= H (possible for 32 < H < 64, where the reference's int shift
+//     makes the height mask all 64 bits) are ignored and counted.
+// ---------------------------------------------------------------------------
+inline uint64_t wordMask(int width) { return width >= 64 ? ~0ull : (1ull << width) - 1ull; }
+inline uint64_t shlW(int k, int width) { return (k >= 0 && k < width) ? (1ull << k) : 0ull; }
+
+// calLightHeightRange, FDc/Step5_2_shadowCal.inl:225-300
+inline uint64_t ctaHeightRange(float A, float B, float C, int W, int H, int width, uint64_t fullH) {
+    const float D = std::fmax(B * (float)(W - 1) + C, C);
+    if (A <= 0.0f && D <= 0.0f) return 0;
+    if (A == 0.0f || D == 0.0f) return fullH;
+    const int k = (int)std::fmax(std::fmin(std::ceil(-1.0f * D / A), (float)H), 0.0f);   // NaN -> H
+    const uint64_t a = (shlW(k, width) - 1ull) & wordMask(width);
+    return A < 0.0f ? a : (~a & fullH);
+}
+
+// calLightRowRange, FDc/Step5_2_shadowCal.inl:302-357
+inline uint64_t ctaRowRange(float B, float D, int W, int width) {
+    if (B <= 0.0f && D <= 0.0f) return 0;
+    if (B == 0.0f || D == 0.0f) return (shlW(W, width) - 1ull) & wordMask(width);
+    const int k = (int)std::fmax(0.0f, std::fmin((float)W, std::ceil(-1.0f * D / B)));   // NaN -> W
+    const uint64_t a = (shlW(k, width) - 1ull) & wordMask(width);
+    return B < 0.0f ? a : (~a & wordMask(width));
+}
+
+// One receiver of shadowCal_CTA_kernel (the body of the sample loop,
+// :682-1046, for every triangle group of the bin's list).
+int occludeCTA(const Light& L, float eps, f3 S, f4 sp, float spz, const int* tris, int ntri,
+               const std::vector<TriData>& td, uint64_t* mask, OccStats& st) {
+    const int W = L.resW, H = L.resH;
+    const int width = (W <= 32) ? 32 : 64;                       // U32 / U64 (:1120-1125)
+    const uint64_t FULL = wordMask(W);                           // U32/U64lightWidthBit (FD/api.cpp:318,325)
+    const uint64_t fullH = (H >= 32) ? wordMask(width) : (1ull << H) - 1ull;   // (T)(1 << H) - 1 (:431)
+    const uint64_t rowsH = wordMask(H);
+    const f3 xd = L.x_delt, yd = L.y_delt, ur = L.upRight;
+    for (int r = 0; r < H; ++r) mask[r] = 0;
+    struct Lane { float A[4], B[4], C[4]; uint64_t rows; };
+    Lane lane[32];
+    for (int g0 = 0; g0 < ntri; g0 += 32) {
+        const int n = std::min(32, ntri - g0);
+        bool depth = false;
+        for (int l = 0; l < n; ++l) {
+            const TriData& t = td[tris[g0 + l]];
+            Lane& c = lane[l];
+            c.rows = 0;
+            ++st.cand;
+            const bool inAABB = t.aabbMin.x <= sp.x && t.aabbMin.y <= sp.y && sp.z <= t.aabbMax.x &&
+                                sp.w <= t.aabbMax.y && t.minz <= spz;   // :686-689
+            if (!inAABB) continue;   // initHeightBit = 0 (:783-784): no rows
+            ++st.pass;
+            const f3 v0 = t.v0, v1 = t.v1, v2 = t.v2;
+            // triangle plane (:667-674)
+            const f3 v1_v2 = v1 - v2, v2_v0 = v2 - v0;
+            const f3 nn = cross(v1_v2, v2_v0);
+            const f3 L_v0 = ur - v0;
+            c.A[3] = dot(nn, yd); c.B[3] = dot(nn, xd); c.C[3] = dot(nn, L_v0);
+            // edge planes (:720-742)
+            const f3 v0_v1 = v0 - v1;
+            const f3 L_v1 = ur - v1, L_v2 = ur - v2;
+            const f3 s_v1 = S - v1;
+            c.A[0] = dot(cross(v0_v1, yd), s_v1); c.B[0] = dot(cross(v0_v1, xd), s_v1); c.C[0] = dot(cross(v0_v1, L_v1), s_v1);
+            const f3 s_v2 = S - v2;
+            c.A[1] = dot(cross(v1_v2, yd), s_v2); c.B[1] = dot(cross(v1_v2, xd), s_v2); c.C[1] = dot(cross(v1_v2, L_v2), s_v2);
+            const f3 s_v0 = S - v0;
+            c.A[2] = dot(cross(v2_v0, yd), s_v0); c.B[2] = dot(cross(v2_v0, xd), s_v0); c.C[2] = dot(cross(v2_v0, L_v0), s_v0);
+            const float temp = dot(nn, s_v0);   // :772
+            if (std::fabs(temp) < eps && pointInTri(v0, v1, v2, S)) continue;   // :783-784
+            ++st.plane;
+            if (temp > 0.0f)   // :785-802
+                for (int k = 0; k < 4; ++k) { c.A[k] *= -1.0f; c.B[k] *= -1.0f; c.C[k] *= -1.0f; }
+            uint64_t rows = fullH;   // :821-842
+            for (int k = 0; k < 4; ++k) rows &= ctaHeightRange(c.A[k], c.B[k], c.C[k], W, H, width, fullH);
+            c.rows = rows;
+            if (rows != 0 && !(t.maxz < spz)) depth = true;   // :876, :929
+        }
+        for (int l = 0; l < n; ++l) {
+            const Lane& c = lane[l];
+            if (c.rows & ~rowsH) ++st.rowClip;
+            uint64_t rows = c.rows & rowsH;
+            while (rows) {   // :880-899 / :933-953 (same rows either way)
+                const int i = __builtin_ctzll(rows);
+                ++st.rows;
+                const float fi = (float)i;
+                uint64_t m = FULL;   // calLightRowShadowWith(Out)Depth, :359-397
+                for (int k = 0; k < 3; ++k) m &= ctaRowRange(c.B[k], c.A[k] * fi + c.C[k], W, width);
+                if (depth) m &= ctaRowRange(c.B[3], c.A[3] * fi + c.C[3], W, width);
+                mask[i] |= m & FULL;
+                rows &= rows - 1ull;
+            }
+        }
     }
     int cnt = 0;
     for (int r = 0; r < H; ++r) cnt += __builtin_popcountll(mask[r]);
@@ -713,8 +823,10 @@ void run(const Input& in, Result& R) {
         OccStats st;
         uint64_t mk[64];
         int ts = R.binTriStart[bin], te = R.binTriEnd[bin];
-        int cnt = occludeOne(L, eps, P(id), sp, spz, R.pairTri.data() + ts, te - ts, R.triData, mk, st);
-        R.shadow[id] = (float)cnt / (float)L.resNum;  // :231
+        int cnt = in.variant == 1
+                      ? occludeCTA(L, eps, P(id), sp, spz, R.pairTri.data() + ts, te - ts, R.triData, mk, st)
+                      : occludeOne(L, eps, P(id), sp, spz, R.pairTri.data() + ts, te - ts, R.triData, mk, st);
+        R.shadow[id] = (float)cnt / (float)L.resNum;  // :231 (Step5_2_shadowCal.inl:1087 for the variant)
         uint32_t* out = &R.masks[(size_t)id * LH * R.wpr];
         for (int rr = 0; rr < LH; ++rr) {
             if (R.wpr == 1) out[rr] = (uint32_t)mk[rr];
@@ -751,6 +863,7 @@ struct or_input {
     int grid_log2;
     const int* subset; int nsubset;
     int nthreads;
+    int variant;
 };
 
 void* or_run(const or_input* ci) {
@@ -767,6 +880,7 @@ void* or_run(const or_input* ci) {
     in.light.resW = ci->light.res_w; in.light.resH = ci->light.res_h;
     in.eps = ci->eps; in.backfaceCull = ci->backface_cull; in.gridLog2 = ci->grid_log2;
     in.subset = ci->subset; in.nsubset = ci->nsubset; in.nthreads = ci->nthreads;
+    in.variant = ci->variant;
     auto* R = new orc::Result();
     orc::run(in, *R);
     return R;
@@ -878,6 +992,15 @@ int or_occlude_all(void* h, const or_input* ci, const int* ids, int n, uint32_t*
         }
     }
     return 0;
+}
+
+// The bin-queue variant's range primitives on their own (closed-form tests):
+// kind 0 = calLightRowRange(B = a, D = b), kind 1 = calLightHeightRange(A = a, B = b, C = c).
+unsigned long long or_cta_range(int kind, float a, float b, float c, int lightW) {
+    const int width = lightW <= 32 ? 32 : 64;
+    if (kind == 0) return orc::ctaRowRange(a, b, lightW, width);
+    const uint64_t fullH = (lightW >= 32) ? orc::wordMask(width) : (1ull << lightW) - 1ull;
+    return orc::ctaHeightRange(a, b, c, lightW, lightW, width, fullH);
 }
 
 // The light-row mask primitive on its own (for closed-form tests).

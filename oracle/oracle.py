@@ -37,7 +37,7 @@ class _Input(ctypes.Structure):
                 ("pos4", ctypes.c_void_p), ("nrm4", ctypes.c_void_p), ("w", ctypes.c_int), ("h", ctypes.c_int),
                 ("light", _Light), ("eps", ctypes.c_float), ("backface_cull", ctypes.c_int),
                 ("grid_log2", ctypes.c_int), ("subset", ctypes.c_void_p), ("nsubset", ctypes.c_int),
-                ("nthreads", ctypes.c_int)]
+                ("nthreads", ctypes.c_int), ("variant", ctypes.c_int)]
 
 
 TRI_DATA_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), ("maxz", "<f4"),
@@ -73,6 +73,8 @@ def lib():
         L.or_rowmask.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_int)]
         L.or_rowmask.restype = ctypes.c_ulonglong
+        L.or_cta_range.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.or_cta_range.restype = ctypes.c_ulonglong
         L.or_make_rectangle.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.or_shade.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 6
         L.or_shade.restype = None
@@ -156,9 +158,10 @@ def occlude_all(result, frame, ids):
     return out
 
 
-def run(frame, subset=None, nthreads=0):
+def run(frame, subset=None, nthreads=0, variant=0):
     """Run the oracle on a workloads.Frame.  `subset`: dense receiver ids for
-    which the occlusion stage is evaluated (None = all)."""
+    which the occlusion stage is evaluated (None = all).  `variant`: 0 the
+    per-sample occlusion kernel, 1 the bin-queue (CTA) kernel."""
     sc, rx, L = frame.scene, frame.rx, frame.light
     verts = np.ascontiguousarray(sc.verts, np.float32)
     idx = np.ascontiguousarray(sc.idx, np.int32)
@@ -185,6 +188,7 @@ def run(frame, subset=None, nthreads=0):
         inp.subset = sub.ctypes.data
         inp.nsubset = sub.shape[0]
     inp.nthreads = nthreads
+    inp.variant = variant
     h = lib().or_run(ctypes.byref(inp))
     return OracleResult(h)
 
@@ -193,6 +197,16 @@ def rowmask(D, B, n, light_w):
     clip = ctypes.c_int()
     m = lib().or_rowmask(D, B, n, light_w, ctypes.byref(clip))
     return int(m), bool(clip.value)
+
+
+def cta_row_range(B, D, light_w):
+    """calLightRowRange of the bin-queue kernel (Step5_2_shadowCal.inl:302-357)."""
+    return int(lib().or_cta_range(0, B, D, 0.0, light_w))
+
+
+def cta_height_range(A, B, C, light_w):
+    """calLightHeightRange of the bin-queue kernel (Step5_2_shadowCal.inl:225-300), H = W."""
+    return int(lib().or_cta_range(1, A, B, C, light_w))
 
 
 def make_rectangle(mat16, p3):

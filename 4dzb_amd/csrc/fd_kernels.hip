@@ -1421,24 +1421,33 @@ __device__ __forceinline__ int chunk_rank(const FdArgs& a, int c, int nb) {
 template <typename T>
 __device__ __forceinline__ T cta_shl(int k) { return (k >= 0 && k < Bits<T>::width) ? ((T)1 << k) : (T)0; }
 
-// calLightRowRange (:302-357); bits >= W are cleared by the caller
+// min(max(ceil(-D / div), 0), R) with the IEEE quotient (NaN -> R), from
+// the reciprocal r = rcp_or_nan(div) where k_fast allows it (its bound holds
+// for this clamp as well: beyond R + 1 both give R, below 0 both give 0).
+__device__ __forceinline__ int cta_k(float D, float div, float r, float Rf) {
+    bool slow;
+    int k = k_fast(D, r, Rf, slow);
+    if (slow) k = (int)fmaxf(0.0f, fminf(Rf, ceilf(-1.0f * D / div)));
+    return k;
+}
+
+// calLightRowRange (:302-357); bits >= W are cleared by the caller.
+// rB = rcp_or_nan(B).
 template <typename T>
-__device__ __forceinline__ T cta_row_range(float B, float D, int W) {
+__device__ __forceinline__ T cta_row_range(float B, float D, int W, float rB) {
     if (B <= 0.0f && D <= 0.0f) return (T)0;
     if (B == 0.0f || D == 0.0f) return cta_shl<T>(W) - (T)1;
-    const int k = (int)fmaxf(0.0f, fminf((float)W, ceilf(-1.0f * D / B)));   // NaN -> W
-    const T m = cta_shl<T>(k) - (T)1;
+    const T m = cta_shl<T>(cta_k(D, B, rB, (float)W)) - (T)1;
     return B < 0.0f ? m : ~m;
 }
 
-// calLightHeightRange (:225-300), H = W = R
+// calLightHeightRange (:225-300), H = W = R; rA = rcp_or_nan(A)
 template <typename T>
-__device__ __forceinline__ T cta_height_range(float A, float B, float C, int R, T fullH) {
+__device__ __forceinline__ T cta_height_range(float A, float B, float C, int R, T fullH, float rA) {
     const float D = fmaxf(B * (float)(R - 1) + C, C);
     if (A <= 0.0f && D <= 0.0f) return (T)0;
     if (A == 0.0f || D == 0.0f) return fullH;
-    const int k = (int)fmaxf(fminf(ceilf(-1.0f * D / A), (float)R), 0.0f);   // NaN -> R
-    const T m = cta_shl<T>(k) - (T)1;
+    const T m = cta_shl<T>(cta_k(D, A, rA, (float)R)) - (T)1;
     return A < 0.0f ? m : (~m & fullH);
 }
 
@@ -1506,7 +1515,7 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
             if (!__any(in)) continue;
             const float S0 = bcast(q.q[0], s), S1 = bcast(q.q[1], s), S2 = bcast(q.q[2], s);
             T rows = 0;
-            float A[4], B[4], C[4];
+            float A[4], B[4], C[4], rB[4];
             if (in) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -1526,7 +1535,10 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
                     }
                     rows = fullH;   // :821-842
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) rows &= cta_height_range<T>(A[k], B[k], C[k], R, fullH);
+                    for (int k = 0; k < 4; ++k) {
+                        rows &= cta_height_range<T>(A[k], B[k], C[k], R, fullH, rcp_or_nan(A[k]));
+                        rB[k] = rcp_or_nan(B[k]);
+                    }
                 }
             }
             // the group's triangle-plane decision (:876, :929), on the rows
@@ -1539,8 +1551,8 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
                 const float fi = (float)i;
                 T m = FULL;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) m &= cta_row_range<T>(B[k], A[k] * fi + C[k], R);
-                if (depth) m &= cta_row_range<T>(B[3], A[3] * fi + C[3], R);
+                for (int k = 0; k < 3; ++k) m &= cta_row_range<T>(B[k], A[k] * fi + C[k], R, rB[k]);
+                if (depth) m &= cta_row_range<T>(B[3], A[3] * fi + C[3], R, rB[3]);
                 m &= FULL;
                 if (m != (T)0) {
                     T cur;

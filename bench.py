@@ -362,6 +362,32 @@ def gpu_target_config(pkg, steps=10):
                 gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
 
+def gpu_bin_queue(pkg, g, frame, steps=10):
+    """The reference's unused bin-queue occlusion kernel (shadowCal_CTA_kernel)
+    on the same frame: frame time next to the per-sample kernel's, and how
+    far its different arithmetic moves the shadow values."""
+    fd = pkg.fd4d
+    run = g["run"]
+    ctx = run.ctx
+    ctx.set_occlusion_variant(fd.OCC_BIN_QUEUE)
+    rc, st = run.run()
+    if rc != 0:
+        raise RuntimeError(f"bin-queue frame failed: rc {rc}")
+    lat = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        ctx.launch(run.stream_handle)
+        lat.append((time.perf_counter() - a) * 1e3)
+    sh, _ = run.outputs()
+    ctx.set_occlusion_variant(fd.OCC_PER_SAMPLE)
+    d = np.abs(sh.astype(np.float64) - g["shadow"].astype(np.float64))
+    return dict(kernel="shadowCal_CTA_kernel (Step5_2_shadowCal.inl:408-1100)", frame_ms=round(pctl(lat, 50), 3),
+                per_sample_frame_ms=round(pctl(g["lat"], 50), 3),
+                slowdown=round(pctl(lat, 50) / max(pctl(g["lat"], 50), 1e-9), 2),
+                shadow_abs_diff=dict(max=float(d.max()), mean=float(d.mean())),
+                receivers_differing=int(np.count_nonzero(d)))
+
+
 def gpu_shard_tile(pkg, steps=10):
     """One GPU's share of a frame sharded by screen tiles over 8 GPUs: a
     480x540 tile (259,200 receivers, 1/8 of 1920x1080) of the boxes-1080p
@@ -595,6 +621,9 @@ def main():
                frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
                                 valid_triangles=g["stats"]["valid_triangles"]),
                roofline=roofline)
+    if not args.quick:
+        log("[bench] bin-queue occlusion variant (same frame)")
+        out["bin_queue_variant"] = gpu_bin_queue(pkg, g, frame)
     if not args.no_batch and not args.quick:
         log("[bench] batched form (batch-64)")
         out["batched"] = gpu_batched(pkg)

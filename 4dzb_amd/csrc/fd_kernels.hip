@@ -1365,28 +1365,37 @@ __device__ __forceinline__ bool point_in_tri(const float* A, const float* B, con
 }
 
 template <typename T>
-__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, const int* ids, int n, int R, int wpr) {
+__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
     // coalesced write-out: consecutive lanes write consecutive words of one
-    // receiver's mask (R rows x wpr words)
+    // receiver's mask (R rows x wpr words); the receiver id comes from its
+    // lane by a cross-lane read with every lane active
     const int per = R * wpr;
     const int total = n * per;
-    for (int j = lane_id(); j < total; j += 64) {
-        const int l = j / per, w = j - l * per;
-        const int r = w / wpr, h = w - r * wpr;
-        a.masks[(size_t)ids[l] * per + w] = lds[(h * R + r) * 64 + l];
+    for (int j0 = 0; j0 < total; j0 += 64) {
+        const int j = j0 + lane_id();
+        const int l = min(j, total - 1) / per, w = j - l * per;
+        const int idl = __shfl(id, l, 64);
+        if (j < total) {
+            const int r = w / wpr, h = w - r * wpr;
+            a.masks[(size_t)idl * per + w] = lds[(h * R + r) * 64 + l];
+        }
     }
 }
 
 // Merge of a segment's partial masks: atomic OR of the non-zero words into
 // the (zeroed) output; same word order as store_masks.
-__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, const int* ids, int n, int R, int wpr) {
+__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
     const int per = R * wpr;
     const int total = n * per;
-    for (int j = lane_id(); j < total; j += 64) {
-        const int l = j / per, w = j - l * per;
-        const int r = w / wpr, h = w - r * wpr;
-        const uint32_t v = lds[(h * R + r) * 64 + l];
-        if (v) atomicOr(a.masks + (size_t)ids[l] * per + w, v);
+    for (int j0 = 0; j0 < total; j0 += 64) {
+        const int j = j0 + lane_id();
+        const int l = min(j, total - 1) / per, w = j - l * per;
+        const int idl = __shfl(id, l, 64);
+        if (j < total) {
+            const int r = w / wpr, h = w - r * wpr;
+            const uint32_t v = lds[(h * R + r) * 64 + l];
+            if (v) atomicOr(a.masks + (size_t)idl * per + w, v);
+        }
     }
 }
 
@@ -1574,13 +1583,12 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
 
 template <typename T, bool COUNT, bool CTA = false>
 __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
-    extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 ids
+    extern __shared__ uint32_t lds[];  // wpr * R * 64 row words (no other LDS: it bounds the waves per CU)
     const FdState* s = a.st;
     const int status = s->status;
     const int R = a.L.res;                  // rows == columns
     const int wpr = a.L.wpr;
     const int lane = lane_id();
-    int* ids = (int*)(lds + wpr * R * 64);
     const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
     // work item = (chunk c, triangle segment seg); segment-major so that
     // neighbouring work items share a stretch of the same triangle list
@@ -1601,7 +1609,6 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         if (lane < n) {
             back = backStage && __float_as_uint(a.rec[i0 + lane].w) == FD_TAG_BACK;
             a.shadow[i0 + lane] = back ? 1.0f : 0.0f;
-            ids[lane] = i0 + lane;
         }
         for (int r = 0; r < R; ++r) {
             const T v = back ? FULL : (T)0;
@@ -1609,7 +1616,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             if (wpr == 2) lds[(R + r) * 64 + lane] = (uint32_t)((uint64_t)v >> 32);
         }
         __syncthreads();
-        store_masks<T>(a, lds, ids, n, R, wpr);
+        store_masks<T>(a, lds, i0 + lane, n, R, wpr);
         return;
     }
     if (c >= s->totalChunks) return;
@@ -1636,7 +1643,6 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     int id = 0;
     if (act) id = a.sortedId[s0 + lane];
     if ((unsigned)id >= (unsigned)a.WH) id = 0;   // defensive: ids form a permutation
-    ids[lane] = id;
 
     if (bin == nb) {
         // receivers outside the light plane, invalid or back-facing
@@ -1651,7 +1657,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             if (wpr == 2) lds[(R + r) * 64 + lane] = (uint32_t)((uint64_t)v >> 32);
         }
         __syncthreads();
-        store_masks<T>(a, lds, ids, n, R, wpr);
+        store_masks<T>(a, lds, id, n, R, wpr);
         return;
     }
 
@@ -1662,7 +1668,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         if (wpr == 2) lds[(R + r) * 64 + lane] = 0u;
     }
     if (CTA) {
-        T* fullRows = (T*)(ids + 64);
+        T* fullRows = (T*)(lds + wpr * R * 64);
         fullRows[lane] = (T)0;
         __syncthreads();
         cta_tris<T>(a, lds, fullRows, ts, te, n, q, R);
@@ -1674,9 +1680,9 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
             }
             if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;   // :1080-1088
-            store_masks<T>(a, lds, ids, n, R, wpr);
+            store_masks<T>(a, lds, id, n, R, wpr);
         } else {
-            or_masks(a, lds, ids, n, R, wpr);
+            or_masks(a, lds, id, n, R, wpr);
         }
         return;
     }
@@ -1917,8 +1923,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
     }
     __syncthreads();
-    if (S == 1) store_masks<T>(a, lds, ids, n, R, wpr);
-    else or_masks(a, lds, ids, n, R, wpr);
+    if (S == 1) store_masks<T>(a, lds, id, n, R, wpr);
+    else or_masks(a, lds, id, n, R, wpr);
 }
 
 // Split lists only: zeroes the mask output before the occlusion merges into
@@ -2193,7 +2199,7 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
     for (int c = 0; c < 2; ++c) {
         if (p.nocc[c] == 0) continue;
-        const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t) + 64 * sizeof(int);
+        const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t);
         if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
         else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }
@@ -2273,7 +2279,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     // and count them after the merge
     const int chunks = a.chunkSlots * a.segs;
     const int wpr = a.L.wpr;
-    const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * sizeof(int);
+    const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t);
     if (a.segs > 1) k_zero_masks<<<zero_blocks(a), 256, 0, st>>>(a);
     if (a.occVariant == FD_OCC_BIN_QUEUE) {
         const size_t shmCta = shm + 64 * sizeof(uint64_t);   // per-receiver full-row flags

@@ -69,7 +69,16 @@ struct FdArgs {
     int binLds;
     // occlusion kernel: FD_OCC_PER_SAMPLE (the contract) or FD_OCC_BIN_QUEUE
     int occVariant;
+    // per-sample occlusion of 64-bit rows: work items per chunk, each taking
+    // 64 / rowParts rows (1 = all rows in one work item); set at launch
+    int rowParts;
 };
+
+// Row parts of a 64-bit-row work item on the single-frame path: 4 parts of
+// 16 rows keep the LDS per wave at 8 KiB (as for 32-bit rows), so that a CU
+// holds 4x the waves; the plane set-up is repeated per part.  The batched
+// path keeps whole rows.
+#define FD_ROW_PARTS 4
 
 #define FD_LDS_BINS 1024
 #define FD_BIN_LDS_BYTES 20

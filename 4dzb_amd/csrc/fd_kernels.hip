@@ -1364,39 +1364,64 @@ __device__ __forceinline__ bool point_in_tri(const float* A, const float* B, con
     return true;
 }
 
-template <typename T>
-__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
+// Rows rowBase .. rowBase + rows - 1 of the work item's receivers go out;
+// LDS holds them from local row 0, the high words ldsRows rows further.
+__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr,
+                                            int rowBase, int rows, int ldsRows) {
     // coalesced write-out: consecutive lanes write consecutive words of one
-    // receiver's mask (R rows x wpr words); the receiver id comes from its
+    // receiver's mask (rows x wpr words); the receiver id comes from its
     // lane by a cross-lane read with every lane active
+    const int per = R * wpr, perH = rows * wpr;
+    const int total = n * perH;
+    for (int j0 = 0; j0 < total; j0 += 64) {
+        const int j = j0 + lane_id();
+        const int l = min(j, total - 1) / perH, w = j - l * perH;
+        const int idl = __shfl(id, l, 64);
+        if (j < total) {
+            const int r = w / wpr, h = w - r * wpr;
+            a.masks[(size_t)idl * per + rowBase * wpr + w] = lds[(h * ldsRows + r) * 64 + l];
+        }
+    }
+}
+__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
+    store_masks(a, lds, id, n, R, wpr, 0, R, R);
+}
+
+// Masks of n receivers (lane l holds the id of receiver l) set to all
+// occluded (back) or none, written straight to the output.
+__device__ __forceinline__ void fill_masks(const FdArgs& a, int id, bool back, int n, int R, int wpr) {
     const int per = R * wpr;
     const int total = n * per;
+    const uint32_t w0 = (R >= 32) ? 0xffffffffu : ((1u << R) - 1u);
+    const uint32_t w1 = (R >= 64) ? 0xffffffffu : (R > 32 ? (1u << (R - 32)) - 1u : 0u);
     for (int j0 = 0; j0 < total; j0 += 64) {
         const int j = j0 + lane_id();
         const int l = min(j, total - 1) / per, w = j - l * per;
         const int idl = __shfl(id, l, 64);
-        if (j < total) {
-            const int r = w / wpr, h = w - r * wpr;
-            a.masks[(size_t)idl * per + w] = lds[(h * R + r) * 64 + l];
-        }
+        const bool bk = __shfl((int)back, l, 64) != 0;
+        if (j < total) a.masks[(size_t)idl * per + w] = bk ? ((w % wpr) == 0 ? w0 : w1) : 0u;
     }
 }
 
 // Merge of a segment's partial masks: atomic OR of the non-zero words into
 // the (zeroed) output; same word order as store_masks.
-__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
-    const int per = R * wpr;
-    const int total = n * per;
+__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr,
+                                         int rowBase, int rows, int ldsRows) {
+    const int per = R * wpr, perH = rows * wpr;
+    const int total = n * perH;
     for (int j0 = 0; j0 < total; j0 += 64) {
         const int j = j0 + lane_id();
-        const int l = min(j, total - 1) / per, w = j - l * per;
+        const int l = min(j, total - 1) / perH, w = j - l * perH;
         const int idl = __shfl(id, l, 64);
         if (j < total) {
             const int r = w / wpr, h = w - r * wpr;
-            const uint32_t v = lds[(h * R + r) * 64 + l];
-            if (v) atomicOr(a.masks + (size_t)idl * per + w, v);
+            const uint32_t v = lds[(h * ldsRows + r) * 64 + l];
+            if (v) atomicOr(a.masks + (size_t)idl * per + rowBase * wpr + w, v);
         }
     }
+}
+__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
+    or_masks(a, lds, id, n, R, wpr, 0, R, R);
 }
 
 // Work item -> rank of its bin in the occlusion order: binary search over
@@ -1590,14 +1615,24 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const int wpr = a.L.wpr;
     const int lane = lane_id();
     const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
-    // work item = (chunk c, triangle segment seg); segment-major so that
-    // neighbouring work items share a stretch of the same triangle list
-    const int seg = blkIn / a.chunkSlots;
-    const int c = blkIn - seg * a.chunkSlots;
+    // work item = (row part, triangle segment seg, chunk c); segment-major
+    // so that neighbouring work items share a stretch of the same triangle
+    // list.  64-bit rows are split into NH parts of 64 / NH rows (NH work
+    // items per chunk): less LDS per wave, more waves per CU.
     const int S = a.segs;
+    const int NH = (Bits<T>::width == 64 && a.rowParts > 1) ? a.rowParts : 1;
+    const int part = blkIn / (a.chunkSlots * S);
+    const int blkS = blkIn - part * a.chunkSlots * S;
+    const int seg = blkS / a.chunkSlots;
+    const int c = blkS - seg * a.chunkSlots;
+    const int ldsRows = NH > 1 ? 64 / NH : R;             // rows held in LDS (per 32-bit word plane)
+    const int rowBase = part * ldsRows;
+    const int rowsH = NH > 1 ? min(ldsRows, R - rowBase) : R;   // rows of this work item
+    if (rowsH <= 0) return;                                // (a part beyond a grid of fewer than 64 rows)
+    const T partMask = NH > 1 ? (T)((((1ull << (rowsH - 1)) << 1) - 1ull) << (rowBase & 63)) : FULL;
 
     if (status != FD_EXIT_OK) {
-        if (seg != 0) return;
+        if (seg != 0 || part != 0) return;
         // early exit of the frame: every receiver shadow 0, except back-facing
         // receivers once the back-face stage has run (FD/api.cpp:489,
         // Step2_1_samplerRectangleAABB.inl:97)
@@ -1610,20 +1645,14 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             back = backStage && __float_as_uint(a.rec[i0 + lane].w) == FD_TAG_BACK;
             a.shadow[i0 + lane] = back ? 1.0f : 0.0f;
         }
-        for (int r = 0; r < R; ++r) {
-            const T v = back ? FULL : (T)0;
-            lds[r * 64 + lane] = (uint32_t)v;
-            if (wpr == 2) lds[(R + r) * 64 + lane] = (uint32_t)((uint64_t)v >> 32);
-        }
-        __syncthreads();
-        store_masks<T>(a, lds, i0 + lane, n, R, wpr);
+        fill_masks(a, i0 + lane, back, n, R, wpr);
         return;
     }
     if (c >= s->totalChunks) return;
     const int nb = s->nb;
     const int rank = chunk_rank(a, c, nb);
     const int bin = uniform(a.binAtRank[rank]);
-    if (bin == nb && seg != 0) return;
+    if (bin == nb && (seg != 0 || part != 0)) return;
     // this work item's stretch of the bin's triangle list (bins 0..nb-1)
     int ts = 0, te = 0;
     if (bin != nb) {
@@ -1651,21 +1680,15 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             back = __float_as_uint(a.rec[id].w) == FD_TAG_BACK;
             a.shadow[id] = back ? 1.0f : 0.0f;
         }
-        for (int r = 0; r < R; ++r) {
-            const T v = back ? FULL : (T)0;
-            lds[r * 64 + lane] = (uint32_t)v;
-            if (wpr == 2) lds[(R + r) * 64 + lane] = (uint32_t)((uint64_t)v >> 32);
-        }
-        __syncthreads();
-        store_masks<T>(a, lds, id, n, R, wpr);
+        fill_masks(a, id, back, n, R, wpr);
         return;
     }
 
     FdRxRec q{};
     if (act) q = a.sortedRec[s0 + lane];
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < ldsRows; ++r) {
         lds[r * 64 + lane] = 0u;
-        if (wpr == 2) lds[(R + r) * 64 + lane] = 0u;
+        if (wpr == 2) lds[(ldsRows + r) * 64 + lane] = 0u;
     }
     if (CTA) {
         T* fullRows = (T*)(lds + wpr * R * 64);
@@ -1680,7 +1703,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
             }
             if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;   // :1080-1088
-            store_masks<T>(a, lds, id, n, R, wpr);
+            store_masks(a, lds, id, n, R, wpr);
         } else {
             or_masks(a, lds, id, n, R, wpr);
         }
@@ -1717,7 +1740,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     for (int ti = ts; ti < te; ++ti) {
         const uint64_t liveMask = __ballot(live);
         if (liveMask == 0) break;
-        if (COUNT) { nCand += __popcll(liveMask); ++nWaveIt; }
+        if (COUNT && part == 0) { nCand += __popcll(liveMask); ++nWaveIt; }
         const int tid = triList[ti];
         if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
@@ -1725,7 +1748,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         const bool pass = live && t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx &&
                           q.rmaxy <= t.bmaxy && t.minz <= q.z;
         if (!pass) continue;
-        if (COUNT) ++nPass;
+        if (COUNT && part == 0) ++nPass;
         // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172) start from
         // a_k = v_{k+1} - Q; a_2 = v0 - Q also gives
         // dir = dot(Q - v0, n) (:130-132) = -dot(v0 - Q, n), exactly (every
@@ -1789,7 +1812,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fabsf(e[2].A)),
                                 fmaxf(fmaxf(fabsf(e[0].B), fabsf(e[1].B)), fabsf(e[2].B)));
         const bool big = !(mAB <= 1.2676506e+30f);
-        if (COUNT) {
+        if (COUNT && part == 0) {
             // the rows the reference visits for this pair: its row-range mask
             // minus the rows already full (general form, before any window)
             ++nPlanes;
@@ -1816,13 +1839,13 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) D[k] = e[k].A * fr + e[k].C;
                 const T M = rowMask(D);
-                uint32_t* w0 = lds + r * 64 + lane;
+                uint32_t* w0 = lds + (r - rowBase) * 64 + lane;
                 T cur;
                 if (Bits<T>::width == 32) {
                     cur = (T)(*w0 | (uint32_t)M);
                     *w0 = (uint32_t)cur;
                 } else {
-                    uint32_t* w1 = lds + (R + r) * 64 + lane;
+                    uint32_t* w1 = lds + (ldsRows + r - rowBase) * 64 + lane;
                     const uint64_t mm = (uint64_t)M;
                     const uint32_t lo32 = *w0 | (uint32_t)mm, hi32 = *w1 | (uint32_t)(mm >> 32);
                     *w0 = lo32;
@@ -1859,7 +1882,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             // rows that may hold occluded samples (:176-191), then the row
             // window; a wave none of whose lanes keeps a row skips the
             // triangle at either step (the cheaper test first)
-            T valid = (~full) & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
+            T valid = (~full) & partMask & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
             if (!__any(valid != (T)0)) continue;
             valid &= row_window<T>(e, Rf, mAB);
             if (!__any(valid != (T)0)) continue;   // no lane can gain a bit from this triangle
@@ -1885,14 +1908,14 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 e[k].rB = rcp_or_nan(e[k].B);
                 DN[k] = (e[k].B > 0 ? -e[k].nColN : 0.0f) + e[k].C;
             }
-            const T valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
+            const T valid = (~full) & partMask & three_edges<T, true>(e, DN, Rf, R);
             wave_rows(valid);
             rows(valid, [&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
         }
-        if (full == FULL) live = false;
+        if (full == partMask) live = false;
     }
-    // fraction = popcount / (W*H) (:226-231); split lists: after the merge
-    if (S == 1) {
+    // fraction = popcount / (W*H) (:226-231); split lists or rows: after the merge
+    if (S == 1 && NH == 1) {
         int cnt = 0;
         for (int r = 0; r < R; ++r) {
             cnt += __popc(lds[r * 64 + lane]);
@@ -1923,8 +1946,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
     }
     __syncthreads();
-    if (S == 1) store_masks<T>(a, lds, id, n, R, wpr);
-    else or_masks(a, lds, id, n, R, wpr);
+    if (S == 1) store_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows);
+    else or_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows);
 }
 
 // Split lists only: zeroes the mask output before the occlusion merges into
@@ -1940,6 +1963,7 @@ __device__ __forceinline__ void zero_masks_body(const FdArgs& a, const int blk) 
 // Split lists only: shadow value of every receiver of a real bin from its
 // merged mask (popcount / samples, :226-231).
 __device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c) {
+    __shared__ int cnt[64];
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK || c >= s->totalChunks) return;
     const int nb = s->nb;
@@ -1949,14 +1973,25 @@ __device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c
     const int s0 = a.rxStart[bin] + (c - a.chunkStart[rank]) * 64;
     const int n = min(64, a.rxEnd[bin] - s0);
     const int lane = lane_id();
-    if (lane >= n) return;
-    const int id = a.sortedId[s0 + lane];
-    if ((unsigned)id >= (unsigned)a.WH) return;
+    int id = lane < n ? a.sortedId[s0 + lane] : 0;
+    if ((unsigned)id >= (unsigned)a.WH) id = 0;   // defensive: ids form a permutation
+    cnt[lane] = 0;
+    __syncthreads();
+    // each receiver's words are contiguous: consecutive lanes read
+    // consecutive words, and the counts gather per receiver in LDS
     const int per = a.L.res * a.L.wpr;
-    const uint32_t* m = a.masks + (size_t)id * per;
-    int cnt = 0;
-    for (int w = 0; w < per; ++w) cnt += __popc(m[w]);
-    a.shadow[id] = (float)cnt / (float)a.L.resNum;
+    const int total = n * per;
+    for (int j0 = 0; j0 < total; j0 += 64) {
+        const int j = j0 + lane;
+        const int l = min(j, total - 1) / per, w = j - l * per;
+        const int idl = __shfl(id, l, 64);
+        if (j < total) {
+            const int b = __popc(a.masks[(size_t)idl * per + w]);
+            if (b) atomicAdd(&cnt[l], b);
+        }
+    }
+    __syncthreads();
+    if (lane < n) a.shadow[id] = (float)cnt[lane] / (float)a.L.resNum;
 }
 
 // ==========================================================================
@@ -2277,22 +2312,28 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     mark(4);
     // occlusion (or the early-exit fill); split lists zero the masks first
     // and count them after the merge
-    const int chunks = a.chunkSlots * a.segs;
     const int wpr = a.L.wpr;
-    const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t);
     if (a.segs > 1) k_zero_masks<<<zero_blocks(a), 256, 0, st>>>(a);
     if (a.occVariant == FD_OCC_BIN_QUEUE) {
-        const size_t shmCta = shm + 64 * sizeof(uint64_t);   // per-receiver full-row flags
+        const int chunks = a.chunkSlots * a.segs;
+        const size_t shmCta = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * sizeof(uint64_t);   // + full-row flags
         if (wpr == 1) k_occlude_cta<uint32_t><<<chunks, 64, shmCta, st>>>(a);
         else k_occlude_cta<uint64_t><<<chunks, 64, shmCta, st>>>(a);
-    } else if (a.opCount) {
-        if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64, shm, st>>>(a);
-        else k_occlude<uint64_t, true><<<chunks, 64, shm, st>>>(a);
     } else {
-        if (wpr == 1) k_occlude<uint32_t, false><<<chunks, 64, shm, st>>>(a);
-        else k_occlude<uint64_t, false><<<chunks, 64, shm, st>>>(a);
+        // 64-bit rows: FD_ROW_PARTS work items per chunk, 64 / FD_ROW_PARTS rows each
+        FdArgs ao = a;
+        ao.rowParts = wpr == 2 ? FD_ROW_PARTS : 1;
+        const int chunks = a.chunkSlots * a.segs * ao.rowParts;
+        const size_t shm = (size_t)wpr * (wpr == 2 ? 64 / FD_ROW_PARTS : a.L.res) * 64 * sizeof(uint32_t);
+        if (a.opCount) {
+            if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64, shm, st>>>(ao);
+            else k_occlude<uint64_t, true><<<chunks, 64, shm, st>>>(ao);
+        } else {
+            if (wpr == 1) k_occlude<uint32_t, false><<<chunks, 64, shm, st>>>(ao);
+            else k_occlude<uint64_t, false><<<chunks, 64, shm, st>>>(ao);
+        }
     }
-    if (a.segs > 1) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
+    if (a.segs > 1 || (wpr == 2 && a.occVariant != FD_OCC_BIN_QUEUE)) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
     mark(5);
     return hipGetLastError();
 }

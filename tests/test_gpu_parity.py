@@ -46,6 +46,15 @@ def test_light_grids(fd, oracle, wl, res):
     _check(fd, oracle, fr)
 
 
+@pytest.mark.parametrize("res", [40, 64])
+def test_wide_rows_one_item_per_chunk(fd, oracle, wl, res):
+    """64-bit rows on a frame large enough for unsplit triangle lists: each
+    chunk's rows go to four work items of 16 rows (the last ones empty or
+    partial below 64 rows), merged by plain stores, counted afterwards."""
+    fr = wl.plane_256(ntri=4000, res=384, light_res=res)
+    _check(fd, oracle, fr)
+
+
 def test_non_pow2_view_and_cull_off(fd, oracle, wl):
     fr = wl.plane_256(ntri=3000)
     fr.rx = wl.receivers_heightfield(200, 120, 4.0, 2.4)

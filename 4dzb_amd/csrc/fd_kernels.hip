@@ -1960,8 +1960,8 @@ __device__ __forceinline__ void zero_masks_body(const FdArgs& a, const int blk) 
         if (base + k < n) a.masks[base + k] = 0u;
 }
 
-// Split lists only: shadow value of every receiver of a real bin from its
-// merged mask (popcount / samples, :226-231).
+// Split lists or split 64-bit rows: shadow value of every receiver of a
+// real bin from its merged mask (popcount / samples, :226-231).
 __device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c) {
     __shared__ int cnt[64];
     const FdState* s = a.st;

@@ -84,10 +84,9 @@ struct FdArgs {
 #define FD_BIN_LDS_BYTES 20
 
 // Pair scatter with all 16 waves of a triangle block: 16 x 16-bit per-wave
-// counts and a 32-bit block offset per bin in LDS (144 KiB at 4096 bins);
-// above FD_SCATTER_WIDE_BINS bins one wave per block scatters.
+// counts and a 32-bit block offset per bin in LDS (144 KiB at the largest
+// grid, 64 x 64 = 4096 bins).
 #define FD_SCATTER_WIDE_LDS 36
-#define FD_SCATTER_WIDE_BINS 4096
 
 // Occlusion counters, in the order documented at fd_get_profile.
 #define FD_NOPS FD_PROFILE_COUNTERS

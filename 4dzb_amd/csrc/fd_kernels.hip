@@ -7,7 +7,7 @@
 //   writeSampleAndBinPair + maxZpreBinReduce      -> k_rx_key
 //   thrust::sort_by_key (receivers) + FindStart    -> k_scan_* + k_rx_scatter
 //   triangleVertexSetup + countPairNumForPerTri    -> k_tri_count
-//   exclusive_scan + writeTriBinPair + sort_by_key -> k_scan_* + k_tri_scatter
+//   exclusive_scan + writeTriBinPair + sort_by_key -> k_scan_* + k_tri_write + k_tri_scatter_wide
 //   findTriangleStartEnd                           -> k_tables
 //   shadowCal_per_sample_kernel                    -> k_occlude
 //
@@ -846,62 +846,18 @@ __device__ __forceinline__ void tri_write_body(const FdArgs& a, const int blk) {
 }
 
 // Stable counting scatter of one triangle block's pairs (the reference's
-// stable sort_by_key by bin): one wave walks the block's triangle-major pair
-// list in chunks of 64, lanes with equal bins are matched by ballots over the
-// key bits and ranked with mbcnt, and per-bin running offsets live in LDS
-// (first use: the block's scanned per-bin offset).  Result: bin-major,
-// ascending triangle id within a bin.
-__device__ __forceinline__ void tri_scatter_body(const FdArgs& a, const int blk) {
-    extern __shared__ int cnt[];  // nb running offsets, starting at the block's scanned histogram
-    const FdState* s = a.st;
-    if (s->status != FD_EXIT_OK) return;
-    const int nb = s->nb;
-    int kbits = 1;
-    while ((1 << kbits) < nb) ++kbits;
-    for (int k = threadIdx.x; k < nb; k += 64) cnt[k] = (int)a.histT[(size_t)k * a.nblkT + blk];
-    __syncthreads();
-    const int t0 = blk * FD_TRB;
-    const int t1 = min(t0 + FD_TRB, a.ntris);
-    const unsigned long long p0 = a.triOff[t0];
-    const unsigned long long p1 = t1 < a.ntris ? (unsigned long long)a.triOff[t1] : s->pairs;
-    const unsigned long long cap = a.pairCap;
-    const unsigned long long e = p1 < cap ? p1 : cap;
-    // the next chunk of pairs is loaded while the current one is ranked
-    uint32_t vNext = (p0 + lane_id() < e) ? a.pairKT[p0 + lane_id()] : 0u;
-    for (unsigned long long c0 = p0; c0 < p1; c0 += 64) {
-        const unsigned long long p = c0 + lane_id();
-        const bool act = p < e;
-        const uint32_t v = vNext;
-        const unsigned long long pn = p + 64;
-        vNext = (pn < e) ? a.pairKT[pn] : 0u;
-        const int key = (int)(v & 0xffffu);
-        uint64_t eq = __ballot(act);
-        for (int b = 0; b < kbits; ++b) {
-            const bool bit = (key >> b) & 1;
-            const uint64_t bal = __ballot(act && bit);
-            eq &= bit ? bal : ~bal;
-        }
-        if (act) {
-            const int rank = mbcnt(eq);
-            const int gcount = __popcll(eq);
-            const int b0 = cnt[key];
-            const unsigned long long dst = (unsigned long long)(uint32_t)b0 + (unsigned)rank;
-            if (rank == gcount - 1) cnt[key] = b0 + gcount;
-            if (dst < cap) a.sortedTri[dst] = t0 + (int)(v >> 16);
-        }
-    }
-}
-
-// The same stable scatter with the block's 16 waves working at once: wave w
-// owns triangles t0 + 64w .. t0 + 64w + 63 and with them a contiguous slice
+// stable sort_by_key by bin, Step4_RasterizeTriangleToBin.inl:206-223), by
+// the block's 16 waves at once: wave w owns triangles t0 + 64w .. t0 + 64w + 63 and with them a contiguous slice
 // of the triangle-major pair list.  A triangle has at most one pair per bin,
 // so a wave holds at most 64 pairs of a bin and the 16 per-wave counts of a
 // bin fit 16-bit halves of LDS words (two waves per word, counted with 32-bit
 // LDS adds).  After a barrier every bin's counts become exclusive prefixes
 // over the waves (at most 1024, still 16 bits), and each wave ranks its
-// slice as the single-wave scatter does, from base + prefix.  The destination
-// of every pair is the single-wave scatter's.  LDS: FD_SCATTER_WIDE_LDS bytes
-// per bin.
+// slice from base + prefix: lanes with equal bins are matched by ballots
+// over the key bits and ranked with mbcnt, the last lane of a group advances
+// the running offset.  Result: bin-major, ascending triangle id within a
+// bin.  LDS: FD_SCATTER_WIDE_LDS bytes per bin (bins <= 4096: grids up to
+// 64 x 64).
 __device__ __forceinline__ int wide_half(int w, int k, int nb) { return (((w >> 1) * nb + k) << 1) | (w & 1); }
 
 __device__ __forceinline__ void tri_scatter_wide_body(const FdArgs& a, const int blk) {
@@ -2008,7 +1964,6 @@ __global__ __launch_bounds__(256) void k_rx_key(FdArgs a) { rx_key_body(a, block
 __global__ __launch_bounds__(64) void k_rx_scatter(FdArgs a) { rx_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_count(FdArgs a) { tri_count_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_write(FdArgs a) { tri_write_body(a, blockIdx.x); }
-__global__ __launch_bounds__(64) void k_tri_scatter(FdArgs a) { tri_scatter_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_TRB) void k_tri_scatter_wide(FdArgs a) { tri_scatter_wide_body(a, blockIdx.x); }
 __global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_FIN_THREADS) void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
@@ -2076,7 +2031,6 @@ FD_BATCHED(kb_rx_key, 256, rx_key_body)
 FD_BATCHED(kb_rx_scatter, 64, rx_scatter_body)
 FD_BATCHED(kb_tri_count, FD_TRB, tri_count_body)
 FD_BATCHED(kb_tri_write, FD_TRB, tri_write_body)
-FD_BATCHED(kb_tri_scatter, 64, tri_scatter_body)
 FD_BATCHED(kb_tri_scatter_wide, FD_TRB, tri_scatter_wide_body)
 FD_BATCHED(kb_tables, 1024, tables_body)
 FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
@@ -2225,10 +2179,7 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 2);
         kb_scan_apply<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
         kb_tri_write<<<L(FD_MAP_TRI), FD_TRB, binLds, st>>>(p.inst, p.map[FD_MAP_TRI], n);
-        if (nb <= FD_SCATTER_WIDE_BINS)
-            kb_tri_scatter_wide<<<L(FD_MAP_TRI), FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(p.inst, p.map[FD_MAP_TRI], n);
-        else
-            kb_tri_scatter<<<L(FD_MAP_TRI), 64, nb * sizeof(int), st>>>(p.inst, p.map[FD_MAP_TRI], n);
+        kb_tri_scatter_wide<<<L(FD_MAP_TRI), FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(p.inst, p.map[FD_MAP_TRI], n);
     }
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
@@ -2300,10 +2251,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         e = scan_u32(a.triOff, a.ntris, a.scanPart, &s->pairs, &s->status, st);
         if (e != hipSuccess) return e;
         k_tri_write<<<a.nblkT, FD_TRB, binLds, st>>>(a);
-        if (nb <= FD_SCATTER_WIDE_BINS)
-            k_tri_scatter_wide<<<a.nblkT, FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(a);
-        else
-            k_tri_scatter<<<a.nblkT, 64, nb * sizeof(int), st>>>(a);
+        k_tri_scatter_wide<<<a.nblkT, FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(a);
     } else {
         k_post_pairs<<<1, FD_FIN_THREADS, 0, st>>>(a);
     }

@@ -74,10 +74,9 @@ struct FdArgs {
     int rowParts;
 };
 
-// Row parts of a 64-bit-row work item on the single-frame path: 4 parts of
-// 16 rows keep the LDS per wave at 8 KiB (as for 32-bit rows), so that a CU
-// holds 4x the waves; the plane set-up is repeated per part.  The batched
-// path keeps whole rows.
+// Row parts of a 64-bit-row work item: 4 parts of 16 rows keep the LDS per
+// wave at 8 KiB (as for 32-bit rows), so that a CU holds 4x the waves; the
+// plane set-up is repeated per part.
 #define FD_ROW_PARTS 4
 
 #define FD_LDS_BINS 1024

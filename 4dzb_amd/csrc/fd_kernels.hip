@@ -2185,7 +2185,8 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
     for (int c = 0; c < 2; ++c) {
         if (p.nocc[c] == 0) continue;
-        const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t);
+        // 64-bit rows run in FD_ROW_PARTS parts of 64 / FD_ROW_PARTS rows
+        const size_t shm = (size_t)(c + 1) * (c == 1 ? 64 / FD_ROW_PARTS : p.maxRes[c]) * 64 * sizeof(uint32_t);
         if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
         else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }

@@ -18,6 +18,7 @@ reference's arithmetic.  cpu_baseline times the CPU restatement of the
 reference algorithm on this host (all cores) over a seeded receiver sample.
 """
 import argparse
+import copy
 import importlib
 import json
 import os
@@ -362,6 +363,40 @@ def gpu_target_config(pkg, steps=10):
                 gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
 
+def gpu_autotuned_layout(pkg, frame, steps=20):
+    """The headline frame with the autotuner's layout (bins per side x tile
+    width; the reference fixes 16x16 bins): frames enqueued back to back,
+    as the plain leg's throughput.  Every candidate layout gives the same
+    output bits (the autotuner checks them against the default layout)."""
+    fd = pkg.fd4d
+    strm = fd.Stream()
+    run = fd.FrameRunner(frame, stream=strm)
+    item = dict(pos=run.pos, nrm=run.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=run.shadow,
+                max_pairs=64 << 20)
+    cands = pkg.autotune.CANDIDATES
+    best, _ = run.ctx.autotune(item, cands, 3)
+    run.close()
+    bins, tile = cands[best] if best >= 0 else (1 << frame.grid_log2, frame.tile_width)
+    f2 = copy.copy(frame)
+    f2.grid_log2 = bins.bit_length() - 1
+    f2.tile_width = tile
+    run = fd.FrameRunner(f2, stream=strm)
+    run.run()
+    for _ in range(3):
+        run.ctx.launch_async(strm.handle)
+    strm.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.ctx.launch_async(strm.handle)
+    strm.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    run.close()
+    strm.close()
+    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+    return dict(layout=dict(bins=bins, tile=tile), ms_per_step=round(ms, 4),
+                gbps=round(W * H * R * R / (ms * 1e-3) / 1e9, 3))
+
+
 def gpu_bin_queue(pkg, g, frame, steps=10):
     """The reference's unused bin-queue occlusion kernel (shadowCal_CTA_kernel)
     on the same frame: frame time next to the per-sample kernel's, and how
@@ -621,6 +656,9 @@ def main():
                frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
                                 valid_triangles=g["stats"]["valid_triangles"]),
                roofline=roofline)
+    if not args.quick and not args.bins and not args.tile:
+        log("[bench] autotuned layout (same frame)")
+        out["autotuned_layout"] = gpu_autotuned_layout(pkg, frame)
     if not args.quick:
         log("[bench] bin-queue occlusion variant (same frame)")
         out["bin_queue_variant"] = gpu_bin_queue(pkg, g, frame)

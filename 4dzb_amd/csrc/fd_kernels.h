@@ -77,7 +77,9 @@ struct FdArgs {
 // Row parts of a 64-bit-row work item: 4 parts of 16 rows keep the LDS per
 // wave at 8 KiB (as for 32-bit rows), so that a CU holds 4x the waves; the
 // plane set-up is repeated per part.
+#ifndef FD_ROW_PARTS
 #define FD_ROW_PARTS 4
+#endif
 
 #define FD_LDS_BINS 1024
 #define FD_BIN_LDS_BYTES 20

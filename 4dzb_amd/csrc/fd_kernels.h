@@ -77,6 +77,12 @@ struct FdArgs {
 // Row parts of a 64-bit-row work item: 4 parts of 16 rows keep the LDS per
 // wave at 8 KiB (as for 32-bit rows), so that a CU holds 4x the waves; the
 // plane set-up is repeated per part.
+// Waves per occlusion work item (per-sample kernel): 2 waves share a work
+// item's receivers and LDS rows and split its triangle list.
+#ifndef FD_OCC_WAVES
+#define FD_OCC_WAVES 2
+#endif
+
 #ifndef FD_ROW_PARTS
 #define FD_ROW_PARTS 4
 #endif

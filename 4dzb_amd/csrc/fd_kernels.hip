@@ -1562,7 +1562,10 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
     }
 }
 
-template <typename T, bool COUNT, bool CTA = false>
+// NW waves per work item (1 or 2): with 2, the waves share the work item's
+// receivers and LDS rows and take alternate triangles of the list (rows
+// merged with LDS atomics), so a CU holds twice the waves for the same LDS.
+template <typename T, bool COUNT, bool CTA = false, int NW = 1>
 __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     extern __shared__ uint32_t lds[];  // wpr * R * 64 row words (no other LDS: it bounds the waves per CU)
     const FdState* s = a.st;
@@ -1570,6 +1573,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const int R = a.L.res;                  // rows == columns
     const int wpr = a.L.wpr;
     const int lane = lane_id();
+    const int wv = NW > 1 ? uniform((int)(threadIdx.x >> 6)) : 0;   // wave of the work item
     const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
     // work item = (row part, triangle segment seg, chunk c); segment-major
     // so that neighbouring work items share a stretch of the same triangle
@@ -1588,7 +1592,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const T partMask = NH > 1 ? (T)((((1ull << (rowsH - 1)) << 1) - 1ull) << (rowBase & 63)) : FULL;
 
     if (status != FD_EXIT_OK) {
-        if (seg != 0 || part != 0) return;
+        if (seg != 0 || part != 0 || wv != 0) return;
         // early exit of the frame: every receiver shadow 0, except back-facing
         // receivers once the back-face stage has run (FD/api.cpp:489,
         // Step2_1_samplerRectangleAABB.inl:97)
@@ -1631,6 +1635,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 
     if (bin == nb) {
         // receivers outside the light plane, invalid or back-facing
+        if (wv != 0) return;
         bool back = false;
         if (act) {
             back = __float_as_uint(a.rec[id].w) == FD_TAG_BACK;
@@ -1642,10 +1647,13 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 
     FdRxRec q{};
     if (act) q = a.sortedRec[s0 + lane];
-    for (int r = 0; r < ldsRows; ++r) {
-        lds[r * 64 + lane] = 0u;
-        if (wpr == 2) lds[(ldsRows + r) * 64 + lane] = 0u;
+    if (wv == 0) {
+        for (int r = 0; r < ldsRows; ++r) {
+            lds[r * 64 + lane] = 0u;
+            if (wpr == 2) lds[(ldsRows + r) * 64 + lane] = 0u;
+        }
     }
+    if (NW > 1) __syncthreads();
     if (CTA) {
         T* fullRows = (T*)(lds + wpr * R * 64);
         fullRows[lane] = (T)0;
@@ -1693,7 +1701,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const cint_p triList = (cint_p)(a.sortedTri);
     const ctri_p tris = (ctri_p)(a.tri);
     bool live = act;   // receiver in this wave and not yet fully shadowed
-    for (int ti = ts; ti < te; ++ti) {
+    for (int ti = ts + wv; ti < te; ti += NW) {
         const uint64_t liveMask = __ballot(live);
         if (liveMask == 0) break;
         if (COUNT && part == 0) { nCand += __popcll(liveMask); ++nWaveIt; }
@@ -1798,14 +1806,25 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 uint32_t* w0 = lds + (r - rowBase) * 64 + lane;
                 T cur;
                 if (Bits<T>::width == 32) {
-                    cur = (T)(*w0 | (uint32_t)M);
-                    *w0 = (uint32_t)cur;
+                    if (NW > 1) {
+                        cur = (T)(atomicOr(w0, (uint32_t)M) | (uint32_t)M);
+                    } else {
+                        cur = (T)(*w0 | (uint32_t)M);
+                        *w0 = (uint32_t)cur;
+                    }
                 } else {
                     uint32_t* w1 = lds + (ldsRows + r - rowBase) * 64 + lane;
                     const uint64_t mm = (uint64_t)M;
-                    const uint32_t lo32 = *w0 | (uint32_t)mm, hi32 = *w1 | (uint32_t)(mm >> 32);
-                    *w0 = lo32;
-                    *w1 = hi32;
+                    uint32_t lo32, hi32;
+                    if (NW > 1) {
+                        lo32 = atomicOr(w0, (uint32_t)mm) | (uint32_t)mm;
+                        hi32 = atomicOr(w1, (uint32_t)(mm >> 32)) | (uint32_t)(mm >> 32);
+                    } else {
+                        lo32 = *w0 | (uint32_t)mm;
+                        hi32 = *w1 | (uint32_t)(mm >> 32);
+                        *w0 = lo32;
+                        *w1 = hi32;
+                    }
                     cur = (T)(((uint64_t)hi32 << 32) | lo32);
                 }
                 if (cur == FULL) full |= (T)1 << r;
@@ -1870,15 +1889,6 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
         if (full == partMask) live = false;
     }
-    // fraction = popcount / (W*H) (:226-231); split lists or rows: after the merge
-    if (S == 1 && NH == 1) {
-        int cnt = 0;
-        for (int r = 0; r < R; ++r) {
-            cnt += __popc(lds[r * 64 + lane]);
-            if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
-        }
-        if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
-    }
     if (COUNT) {
         // per-lane counts: nCand is wave-wide already
         unsigned long long p = nPass, r = nRows, rr = nRefRows, pl = nPlanes;
@@ -1894,7 +1904,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             atomicAdd(a.opCount + 0, nCand);
             atomicAdd(a.opCount + 1, p);
             atomicAdd(a.opCount + 2, r);
-            atomicAdd(a.opCount + 3, 1ull);
+            if (wv == 0) atomicAdd(a.opCount + 3, 1ull);
             atomicAdd(a.opCount + 4, nWaveIt);
             atomicAdd(a.opCount + 5, wr);
             atomicAdd(a.opCount + 6, rr);
@@ -1902,6 +1912,16 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
     }
     __syncthreads();
+    if (wv != 0) return;
+    // fraction = popcount / (W*H) (:226-231); split lists or rows: after the merge
+    if (S == 1 && NH == 1) {
+        int cnt = 0;
+        for (int r = 0; r < R; ++r) {
+            cnt += __popc(lds[r * 64 + lane]);
+            if (wpr == 2) cnt += __popc(lds[(R + r) * 64 + lane]);
+        }
+        if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
+    }
     if (S == 1) store_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows);
     else or_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows);
 }
@@ -1985,8 +2005,8 @@ __global__ __launch_bounds__(256) void k_zero_masks(FdArgs a) { zero_masks_body(
 __global__ __launch_bounds__(64) void k_occlude_finish(FdArgs a) { occlude_finish_body(a, blockIdx.x); }
 
 template <typename T, bool COUNT>
-__global__ __launch_bounds__(64) void k_occlude(FdArgs a) {
-    occlude_body<T, COUNT>(a, xcd_block(blockIdx.x, gridDim.x));
+__global__ __launch_bounds__(64 * FD_OCC_WAVES) void k_occlude(FdArgs a) {
+    occlude_body<T, COUNT, false, FD_OCC_WAVES>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 template <typename T>
 __global__ __launch_bounds__(64) void k_occlude_cta(FdArgs a) {
@@ -2037,11 +2057,11 @@ FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
 #undef FD_BATCHED
 
 template <typename T>
-__global__ __launch_bounds__(64) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
+__global__ __launch_bounds__(64 * FD_OCC_WAVES) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
-    occlude_body<T, false>(a, b - map[i]);
+    occlude_body<T, false, false, FD_OCC_WAVES>(a, b - map[i]);
 }
 __global__ __launch_bounds__(256) void kb_zero_masks(const FdArgs* __restrict__ inst, const int* __restrict__ map,
                                                      int n) {
@@ -2187,8 +2207,8 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         if (p.nocc[c] == 0) continue;
         // 64-bit rows run in FD_ROW_PARTS parts of 64 / FD_ROW_PARTS rows
         const size_t shm = (size_t)(c + 1) * (c == 1 ? 64 / FD_ROW_PARTS : p.maxRes[c]) * 64 * sizeof(uint32_t);
-        if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
-        else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
+        if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64 * FD_OCC_WAVES, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
+        else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64 * FD_OCC_WAVES, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }
     if (p.total[FD_MAP_FIN] > 0) kb_occlude_finish<<<L(FD_MAP_FIN), 64, 0, st>>>(p.occ[0], p.map[FD_MAP_FIN], n);
     return hipGetLastError();
@@ -2274,12 +2294,13 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         ao.rowParts = wpr == 2 ? FD_ROW_PARTS : 1;
         const int chunks = a.chunkSlots * a.segs * ao.rowParts;
         const size_t shm = (size_t)wpr * (wpr == 2 ? 64 / FD_ROW_PARTS : a.L.res) * 64 * sizeof(uint32_t);
+        const int threads = 64 * FD_OCC_WAVES;
         if (a.opCount) {
-            if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64, shm, st>>>(ao);
-            else k_occlude<uint64_t, true><<<chunks, 64, shm, st>>>(ao);
+            if (wpr == 1) k_occlude<uint32_t, true><<<chunks, threads, shm, st>>>(ao);
+            else k_occlude<uint64_t, true><<<chunks, threads, shm, st>>>(ao);
         } else {
-            if (wpr == 1) k_occlude<uint32_t, false><<<chunks, 64, shm, st>>>(ao);
-            else k_occlude<uint64_t, false><<<chunks, 64, shm, st>>>(ao);
+            if (wpr == 1) k_occlude<uint32_t, false><<<chunks, threads, shm, st>>>(ao);
+            else k_occlude<uint64_t, false><<<chunks, threads, shm, st>>>(ao);
         }
     }
     if (a.segs > 1 || (wpr == 2 && a.occVariant != FD_OCC_BIN_QUEUE)) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);

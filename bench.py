@@ -636,7 +636,7 @@ def main():
                     frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
                     traffic=prof["traffic_bytes"] if prof else None,
                     traffic_source=prof["source"] if prof else None,
-                    valu_busy_profiled=prof["valu_busy"] if prof else None,
+                    valu_issue_profiled=prof.get("valu_issue") if prof else None,
                     kernel="k_occlude", kernel_ms=round(occ_ms, 4),
                     work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
                               work_items=int(items), wave_tri_iterations=int(wave_it),

@@ -2004,8 +2004,10 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 __global__ __launch_bounds__(256) void k_zero_masks(FdArgs a) { zero_masks_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_occlude_finish(FdArgs a) { occlude_finish_body(a, blockIdx.x); }
 
+// 8 waves per SIMD (the most a SIMD holds): the target makes the compiler
+// keep the scalar registers within 8 waves' share (else 99-103 SGPRs, 7-8).
 template <typename T, bool COUNT>
-__global__ __launch_bounds__(64 * FD_OCC_WAVES) void k_occlude(FdArgs a) {
+__global__ __launch_bounds__(64 * FD_OCC_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
     occlude_body<T, COUNT, false, FD_OCC_WAVES>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 template <typename T>
@@ -2057,7 +2059,7 @@ FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
 #undef FD_BATCHED
 
 template <typename T>
-__global__ __launch_bounds__(64 * FD_OCC_WAVES) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
+__global__ __launch_bounds__(64 * FD_OCC_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];

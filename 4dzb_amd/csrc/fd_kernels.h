@@ -74,17 +74,21 @@ struct FdArgs {
     int rowParts;
 };
 
-// Row parts of a 64-bit-row work item: 4 parts of 16 rows keep the LDS per
-// wave at 8 KiB (as for 32-bit rows), so that a CU holds 4x the waves; the
-// plane set-up is repeated per part.
-// Waves per occlusion work item (per-sample kernel): 2 waves share a work
-// item's receivers and LDS rows and split its triangle list.
+// Waves per occlusion work item (per-sample kernel), for 32-bit and 64-bit
+// rows: the waves share a work item's receivers and LDS rows and split its
+// triangle list.
 #ifndef FD_OCC_WAVES
-#define FD_OCC_WAVES 2
+#define FD_OCC_WAVES 4
+#endif
+#ifndef FD_OCC_WAVES64
+#define FD_OCC_WAVES64 8
 #endif
 
+// Row parts of a 64-bit-row work item (each part a work item of 64 / parts
+// rows, repeating the plane set-up).  With FD_OCC_WAVES64 waves sharing a
+// work item's LDS, whole rows (1 part) are fastest.
 #ifndef FD_ROW_PARTS
-#define FD_ROW_PARTS 4
+#define FD_ROW_PARTS 1
 #endif
 
 #define FD_LDS_BINS 1024

@@ -2004,11 +2004,17 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 __global__ __launch_bounds__(256) void k_zero_masks(FdArgs a) { zero_masks_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_occlude_finish(FdArgs a) { occlude_finish_body(a, blockIdx.x); }
 
+// Waves per occlusion work item by row width (they share its receivers and
+// LDS rows): the 8 KiB of 32-bit rows of 64 receivers go to 4 waves, the
+// 32 KiB of 64-bit rows to 8, so that LDS never bounds the waves per CU.
+template <typename T>
+constexpr int fd_occ_waves() { return Bits<T>::width == 64 ? FD_OCC_WAVES64 : FD_OCC_WAVES; }
+
 // 8 waves per SIMD (the most a SIMD holds): the target makes the compiler
 // keep the scalar registers within 8 waves' share (else 99-103 SGPRs, 7-8).
 template <typename T, bool COUNT>
-__global__ __launch_bounds__(64 * FD_OCC_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
-    occlude_body<T, COUNT, false, FD_OCC_WAVES>(a, xcd_block(blockIdx.x, gridDim.x));
+__global__ __launch_bounds__(64 * fd_occ_waves<T>()) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
+    occlude_body<T, COUNT, false, fd_occ_waves<T>()>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 template <typename T>
 __global__ __launch_bounds__(64) void k_occlude_cta(FdArgs a) {
@@ -2059,11 +2065,11 @@ FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
 #undef FD_BATCHED
 
 template <typename T>
-__global__ __launch_bounds__(64 * FD_OCC_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
+__global__ __launch_bounds__(64 * fd_occ_waves<T>()) __attribute__((amdgpu_waves_per_eu(8))) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
-    occlude_body<T, false, false, FD_OCC_WAVES>(a, b - map[i]);
+    occlude_body<T, false, false, fd_occ_waves<T>()>(a, b - map[i]);
 }
 __global__ __launch_bounds__(256) void kb_zero_masks(const FdArgs* __restrict__ inst, const int* __restrict__ map,
                                                      int n) {
@@ -2209,8 +2215,8 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         if (p.nocc[c] == 0) continue;
         // 64-bit rows run in FD_ROW_PARTS parts of 64 / FD_ROW_PARTS rows
         const size_t shm = (size_t)(c + 1) * (c == 1 ? 64 / FD_ROW_PARTS : p.maxRes[c]) * 64 * sizeof(uint32_t);
-        if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64 * FD_OCC_WAVES, shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
-        else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64 * FD_OCC_WAVES, shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
+        if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64 * fd_occ_waves<uint32_t>(), shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
+        else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64 * fd_occ_waves<uint64_t>(), shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }
     if (p.total[FD_MAP_FIN] > 0) kb_occlude_finish<<<L(FD_MAP_FIN), 64, 0, st>>>(p.occ[0], p.map[FD_MAP_FIN], n);
     return hipGetLastError();
@@ -2296,7 +2302,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         ao.rowParts = wpr == 2 ? FD_ROW_PARTS : 1;
         const int chunks = a.chunkSlots * a.segs * ao.rowParts;
         const size_t shm = (size_t)wpr * (wpr == 2 ? 64 / FD_ROW_PARTS : a.L.res) * 64 * sizeof(uint32_t);
-        const int threads = 64 * FD_OCC_WAVES;
+        const int threads = 64 * (wpr == 2 ? fd_occ_waves<uint64_t>() : fd_occ_waves<uint32_t>());
         if (a.opCount) {
             if (wpr == 1) k_occlude<uint32_t, true><<<chunks, threads, shm, st>>>(ao);
             else k_occlude<uint64_t, true><<<chunks, threads, shm, st>>>(ao);
@@ -2305,7 +2311,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
             else k_occlude<uint64_t, false><<<chunks, threads, shm, st>>>(ao);
         }
     }
-    if (a.segs > 1 || (wpr == 2 && a.occVariant != FD_OCC_BIN_QUEUE)) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
+    if (a.segs > 1 || (wpr == 2 && FD_ROW_PARTS > 1 && a.occVariant != FD_OCC_BIN_QUEUE)) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
     mark(5);
     return hipGetLastError();
 }

@@ -1056,10 +1056,11 @@ __device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) 
 
 // --------------------------------------------------------------------------
 // Occlusion (shadowCal_per_sample_kernel, Step5_2_shadowCalPerSample.inl:37-234).
-// One wave per work item: 64 sorted receivers of one bin, one receiver per
-// lane.  The bin's triangle list is walked with a wave-uniform index, so
-// triangle records arrive through scalar loads; each lane keeps its light
-// rows in LDS (row-major by row, lane-minor: conflict-free).
+// A work item is 64 sorted receivers of one bin, one receiver per lane, run
+// by several waves that take interleaved triangles of the bin's list.  Each
+// wave walks its triangles with a wave-uniform index, so triangle records
+// arrive through scalar loads; the receivers' light rows live in LDS
+// (row-major by row, lane-minor: conflict-free), shared by the waves.
 // --------------------------------------------------------------------------
 template <typename T>
 struct Bits;

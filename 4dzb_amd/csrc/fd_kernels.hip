@@ -2008,14 +2008,17 @@ __global__ __launch_bounds__(64) void k_occlude_finish(FdArgs a) { occlude_finis
 // Waves per occlusion work item by row width (they share its receivers and
 // LDS rows): the 8 KiB of 32-bit rows of 64 receivers go to 4 waves, the
 // 32 KiB of 64-bit rows to 8, so that LDS never bounds the waves per CU.
-template <typename T>
-constexpr int fd_occ_waves() { return Bits<T>::width == 64 ? FD_OCC_WAVES64 : FD_OCC_WAVES; }
+// The counting variant runs one wave per work item: each receiver then meets
+// its triangles in list order with the full-row state of all earlier ones,
+// as in the reference, so its work counters do not depend on the schedule.
+template <typename T, bool COUNT = false>
+constexpr int fd_occ_waves() { return COUNT ? 1 : (Bits<T>::width == 64 ? FD_OCC_WAVES64 : FD_OCC_WAVES); }
 
 // 8 waves per SIMD (the most a SIMD holds): the target makes the compiler
 // keep the scalar registers within 8 waves' share (else 99-103 SGPRs, 7-8).
 template <typename T, bool COUNT>
-__global__ __launch_bounds__(64 * fd_occ_waves<T>()) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
-    occlude_body<T, COUNT, false, fd_occ_waves<T>()>(a, xcd_block(blockIdx.x, gridDim.x));
+__global__ __launch_bounds__((64 * fd_occ_waves<T, COUNT>())) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
+    occlude_body<T, COUNT, false, fd_occ_waves<T, COUNT>()>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 template <typename T>
 __global__ __launch_bounds__(64) void k_occlude_cta(FdArgs a) {
@@ -2305,8 +2308,8 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         const size_t shm = (size_t)wpr * (wpr == 2 ? 64 / FD_ROW_PARTS : a.L.res) * 64 * sizeof(uint32_t);
         const int threads = 64 * (wpr == 2 ? fd_occ_waves<uint64_t>() : fd_occ_waves<uint32_t>());
         if (a.opCount) {
-            if (wpr == 1) k_occlude<uint32_t, true><<<chunks, threads, shm, st>>>(ao);
-            else k_occlude<uint64_t, true><<<chunks, threads, shm, st>>>(ao);
+            if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64 * fd_occ_waves<uint32_t, true>(), shm, st>>>(ao);
+            else k_occlude<uint64_t, true><<<chunks, 64 * fd_occ_waves<uint64_t, true>(), shm, st>>>(ao);
         } else {
             if (wpr == 1) k_occlude<uint32_t, false><<<chunks, threads, shm, st>>>(ao);
             else k_occlude<uint64_t, false><<<chunks, threads, shm, st>>>(ao);

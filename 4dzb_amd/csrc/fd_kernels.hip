@@ -1324,13 +1324,13 @@ __device__ __forceinline__ bool point_in_tri(const float* A, const float* B, con
 // Rows rowBase .. rowBase + rows - 1 of the work item's receivers go out;
 // LDS holds them from local row 0, the high words ldsRows rows further.
 __device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr,
-                                            int rowBase, int rows, int ldsRows) {
+                                            int rowBase, int rows, int ldsRows, int wave = 0, int waves = 1) {
     // coalesced write-out: consecutive lanes write consecutive words of one
     // receiver's mask (rows x wpr words); the receiver id comes from its
     // lane by a cross-lane read with every lane active
     const int per = R * wpr, perH = rows * wpr;
     const int total = n * perH;
-    for (int j0 = 0; j0 < total; j0 += 64) {
+    for (int j0 = wave * 64; j0 < total; j0 += 64 * waves) {
         const int j = j0 + lane_id();
         const int l = min(j, total - 1) / perH, w = j - l * perH;
         const int idl = __shfl(id, l, 64);
@@ -1363,10 +1363,10 @@ __device__ __forceinline__ void fill_masks(const FdArgs& a, int id, bool back, i
 // Merge of a segment's partial masks: atomic OR of the non-zero words into
 // the (zeroed) output; same word order as store_masks.
 __device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr,
-                                         int rowBase, int rows, int ldsRows) {
+                                         int rowBase, int rows, int ldsRows, int wave = 0, int waves = 1) {
     const int per = R * wpr, perH = rows * wpr;
     const int total = n * perH;
-    for (int j0 = 0; j0 < total; j0 += 64) {
+    for (int j0 = wave * 64; j0 < total; j0 += 64 * waves) {
         const int j = j0 + lane_id();
         const int l = min(j, total - 1) / perH, w = j - l * perH;
         const int idl = __shfl(id, l, 64);
@@ -1648,11 +1648,9 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 
     FdRxRec q{};
     if (act) q = a.sortedRec[s0 + lane];
-    if (wv == 0) {
-        for (int r = 0; r < ldsRows; ++r) {
-            lds[r * 64 + lane] = 0u;
-            if (wpr == 2) lds[(ldsRows + r) * 64 + lane] = 0u;
-        }
+    for (int r = wv; r < ldsRows; r += NW) {
+        lds[r * 64 + lane] = 0u;
+        if (wpr == 2) lds[(ldsRows + r) * 64 + lane] = 0u;
     }
     if (NW > 1) __syncthreads();
     if (CTA) {
@@ -1913,9 +1911,9 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
     }
     __syncthreads();
-    if (wv != 0) return;
-    // fraction = popcount / (W*H) (:226-231); split lists or rows: after the merge
-    if (S == 1 && NH == 1) {
+    // fraction = popcount / (W*H) (:226-231); split lists or rows: after the
+    // merge.  The write-out is shared by the waves.
+    if (wv == 0 && S == 1 && NH == 1) {
         int cnt = 0;
         for (int r = 0; r < R; ++r) {
             cnt += __popc(lds[r * 64 + lane]);
@@ -1923,8 +1921,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
         if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
     }
-    if (S == 1) store_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows);
-    else or_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows);
+    if (S == 1) store_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows, wv, NW);
+    else or_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows, wv, NW);
 }
 
 // Split lists only: zeroes the mask output before the occlusion merges into

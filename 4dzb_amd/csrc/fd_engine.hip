@@ -409,10 +409,14 @@ int fd_set_scene_params(fd_ctx* c, size_t maxTris, size_t maxVerts) {
 
 int fd_set_area_light(fd_ctx* c, const fd_area_light* d) {
     if (!c || !d) return FD_ERR_ARG;
-    if (d->res_w != d->res_h || d->res_w < 1 || d->res_w > 64) return FD_ERR_UNSUPPORTED_LIGHT;
+    // the light replaces the previous one even when its grid is unsupported:
+    // launches then fail with FD_ERR_UNSUPPORTED_LIGHT instead of running the
+    // old light (the reference's kernel reports the grid and writes nothing,
+    // Step5_2_shadowCalPerSample.inl:251-254)
     c->lightDesc = *d;
-    light_setup(*d, c->light);
     c->lightSet = true;
+    if (d->res_w != d->res_h || d->res_w < 1 || d->res_w > 64) return FD_ERR_UNSUPPORTED_LIGHT;
+    light_setup(*d, c->light);
     return FD_OK;
 }
 
@@ -844,6 +848,21 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
 namespace fdk {
 hipError_t debug_rect(const FdArgs& a, float* out8, hipStream_t st);
 hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, int mode, unsigned long long* out);
+hipError_t selftest_rowwindow(const float* coef, int n, int R, unsigned long long* out);
+}
+
+extern "C" int fd_selftest_rowwindow(const float* hcoef, int n, int R, unsigned long long* hout) {
+    if (!hcoef || !hout || n <= 0 || R < 1 || R > 64) return FD_ERR_ARG;
+    float* dC = nullptr;
+    unsigned long long* dO = nullptr;
+    hipError_t e = hipMalloc(&dC, (size_t)n * 9 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dO, (size_t)n * 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemcpy(dC, hcoef, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = fdk::selftest_rowwindow(dC, n, R, dO);
+    if (e == hipSuccess) e = hipMemcpy(hout, dO, (size_t)n * 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (dC) (void)hipFree(dC);
+    if (dO) (void)hipFree(dO);
+    return hip_status(e);
 }
 
 extern "C" int fd_selftest_rowmask(const float* hD, const float* hB, int n, int R, int mode, unsigned long long* hout) {

@@ -2146,6 +2146,62 @@ __global__ void k_selftest_rowmask(const float* D, const float* B, int n, int R,
     }
 }
 
+// Verification of the row window: for edge-plane triple i (coef[9i + 3k ..]
+// = A_k, B_k, C_k) over an R x R grid, out[4i] = the rows whose three-edge
+// AND (general form, clipped to the grid) is non-empty, out[4i + 1] = the
+// same rows by the fast form, out[4i + 2] = row_window's rows, out[4i + 3] =
+// the row-range mask (the rows the reference visits, general form).  Where
+// the occlusion kernel would not take its fast branch (max |A|, |B| > 2^100,
+// the only branch that applies the window) the fast rows repeat the general
+// ones and the window is every row.
+template <typename T>
+__device__ __forceinline__ void selftest_rowwindow_one(const float* coef, int R, unsigned long long* out) {
+    Edge e[3];
+    float nC[3], rB[3], dB[3];
+    const float nsub = (float)(unsigned)(R - 1);
+    const float Rf = (float)R;
+    const float Rs = Rf + 2.44140625e-4f;   // R + 2^-12
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e[k].A = coef[3 * k];
+        e[k].B = coef[3 * k + 1];
+        e[k].C = coef[3 * k + 2];
+        e[k].nRowN = -(e[k].A * nsub);
+        e[k].nColN = -(e[k].B * nsub);
+        e[k].rA = rcp_or_nan(e[k].A);
+        e[k].rB = rcp_or_nan(e[k].B);
+        nC[k] = e[k].nColN;
+        dB[k] = e[k].B;
+        rB[k] = fabsf(e[k].B) >= 7.8886091e-31f ? __builtin_amdgcn_rcpf(e[k].B) : __builtin_nanf("");
+    }
+    const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fabsf(e[2].A)),
+                            fmaxf(fmaxf(fabsf(e[0].B), fabsf(e[1].B)), fabsf(e[2].B)));
+    const bool fast = mAB <= 1.2676506e+30f;
+    const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
+    uint64_t gen = 0, fst = 0;
+    for (int r = 0; r < R; ++r) {
+        float D[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) D[k] = e[k].A * (float)r + e[k].C;
+        if ((FULL & three_edges<T, false>(e, D, Rf, R)) != (T)0) gen |= 1ull << r;
+        if (fast && three_edges_fast<T>(D, nC, rB, dB, Rs, R) != (T)0) fst |= 1ull << r;
+    }
+    float DN[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) DN[k] = (e[k].B > 0 ? -e[k].nColN : 0.0f) + e[k].C;
+    out[0] = gen;
+    out[1] = fast ? fst : gen;
+    out[2] = fast ? (uint64_t)(row_window<T>(e, Rf, mAB) & FULL) : (uint64_t)FULL;
+    out[3] = (uint64_t)(FULL & three_edges<T, true>(e, DN, Rf, R));
+}
+
+__global__ void k_selftest_rowwindow(const float* coef, int n, int R, unsigned long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (R <= 32) selftest_rowwindow_one<uint32_t>(coef + 9 * (size_t)i, R, out + 4 * (size_t)i);
+    else selftest_rowwindow_one<uint64_t>(coef + 9 * (size_t)i, R, out + 4 * (size_t)i);
+}
+
 // Stage read-back of the receiver rectangles in the reference's 2-layer
 // layout (Step2_1_samplerRectangleAABB.inl:122-123); verification only.
 __global__ void k_debug_rect(FdArgs a, float* out) {
@@ -2243,6 +2299,13 @@ hipError_t count_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned l
 
 hipError_t selftest_rowmask(const float* D, const float* B, int n, int R, int mode, unsigned long long* out) {
     k_selftest_rowmask<<<cdiv(n, 256), 256>>>(D, B, n, R, mode, out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+}
+
+hipError_t selftest_rowwindow(const float* coef, int n, int R, unsigned long long* out) {
+    k_selftest_rowwindow<<<cdiv(n, 256), 256>>>(coef, n, R, out);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     return e;

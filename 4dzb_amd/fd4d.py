@@ -145,6 +145,7 @@ def lib():
             "fd_status_string": ([ip], ctypes.c_char_p),
             "fd_set_profiling": ([vp, ip, ip], ip),
             "fd_selftest_rowmask": ([vp, vp, ip, ip, ip, vp], ip),
+            "fd_selftest_rowwindow": ([vp, ip, ip, vp], ip),
             "fd_device_count": ([ctypes.POINTER(ip)], ip),
             "fd_batch_create": ([vp, ctypes.c_void_p, ip, ctypes.POINTER(vp)], ip),
             "fd_batch_capture": ([vp, vp], ip),
@@ -190,6 +191,18 @@ def selftest_rowmask(D, B, res, mode=1):
     out = np.empty(len(D), np.uint64)
     _check(lib().fd_selftest_rowmask(D.ctypes.data, B.ctypes.data, len(D), res, mode, out.ctypes.data),
            "fd_selftest_rowmask")
+    return out
+
+
+def selftest_rowwindow(coef, res):
+    """Device row window on host edge-plane triples (n, 9) float32
+    (verification only).  Returns (n, 4) uint64: non-empty rows by the
+    general form, by the fast form, the window's rows and the row-range
+    mask."""
+    coef = np.ascontiguousarray(coef, np.float32).reshape(-1, 9)
+    out = np.empty((coef.shape[0], 4), np.uint64)
+    _check(lib().fd_selftest_rowwindow(coef.ctypes.data, coef.shape[0], res, out.ctypes.data),
+           "fd_selftest_rowwindow")
     return out
 
 

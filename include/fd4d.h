@@ -146,7 +146,10 @@ int fd_set_scenes_esp(fd_ctx* ctx, float eps);
 int fd_set_scene_params(fd_ctx* ctx, size_t max_triangles, size_t max_vertices);
 
 /* FD::FDsetAreaLight, FD/api.cpp:303-400.  Unlike the reference, launches
- * stay correct without calling this again before every frame. */
+ * stay correct without calling this again before every frame.  A grid that is
+ * not square or outside 1..64 returns FD_ERR_UNSUPPORTED_LIGHT and still
+ * replaces the previous light: launches return FD_ERR_UNSUPPORTED_LIGHT until
+ * a supported light is set. */
 int fd_set_area_light(fd_ctx* ctx, const fd_area_light* light);
 
 /* FD::FDLaunch, FD/api.cpp:402-582.  Runs one frame on `hip_stream` (NULL =
@@ -375,6 +378,17 @@ int fd_debug_read(fd_ctx* ctx, const char* name, void* host_dst, size_t max_byte
  * mode 0: the kernel's general form; mode 1: its fast form wherever that
  * form's precondition (|B| <= 2^100) holds, the general form elsewhere. */
 int fd_selftest_rowmask(const float* D, const float* B, int n, int res, int mode, unsigned long long* out);
+
+/* Verification only: the occlusion kernel's row window (rows outside it are
+ * skipped because no light sample of them can be set; the reference visits
+ * every row of its row-range mask, Step5_2_shadowCalPerSample.inl:191-222).
+ * For n edge-plane triples (coef: n x {A0, B0, C0, A1, B1, C1, A2, B2, C2}),
+ * plane k at row r, column c being A_k r + B_k c + C_k, over a res x res grid,
+ * out receives 4 row masks per triple: the rows whose three-edge mask is
+ * non-empty by the general form, the same by the fast form, the rows of the
+ * window (every row where the kernel would not apply it) and the row-range
+ * mask (the rows the reference visits). */
+int fd_selftest_rowwindow(const float* coef, int n, int res, unsigned long long* out);
 
 const char* fd_status_string(int code);
 

@@ -1069,6 +1069,25 @@ void or_shade(int n, const float* shadow, const float* g0, const float* g1, cons
     }
 }
 
+// FD::moveModel (4DZB/FDZB_Soft_Shadow/api.cpp:17-20) -> cudaModeTransform ->
+// the transform kernel (FDc/moveModel.inl:6-30): each vertex position, with
+// w set to 1, is multiplied by the column-major matrix m (mul, :6-13: each
+// output is ((m[i]*x + m[4+i]*y) + m[8+i]*z) + m[12+i]*w, left to right, no
+// FMA) and x, y, z are written back in place; every other float of the
+// vertex (stride vstride floats) is left as it was.
+void or_move_model(float* verts, int vstride, int nverts, const float* m) {
+    for (int t = 0; t < nverts; ++t) {
+        float* p = verts + (size_t)t * vstride;
+        const float x = p[0], y = p[1], z = p[2], w = 1.0f;   // pos.w = 1.0f (:21)
+        const float rx = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
+        const float ry = m[1] * x + m[5] * y + m[9] * z + m[13] * w;
+        const float rz = m[2] * x + m[6] * y + m[10] * z + m[14] * w;
+        p[0] = rx;
+        p[1] = ry;
+        p[2] = rz;
+    }
+}
+
 int or_max_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

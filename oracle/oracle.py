@@ -79,6 +79,8 @@ def lib():
         L.or_shade.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 6
         L.or_shade.restype = None
         L.or_max_threads.restype = ctypes.c_int
+        L.or_move_model.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.or_move_model.restype = None
         L.or_occlude_all.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Input), ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_void_p]
         L.or_occlude_all.restype = ctypes.c_int
@@ -219,6 +221,16 @@ def make_rectangle(mat16, p3):
 
 def max_threads():
     return lib().or_max_threads()
+
+
+def move_model(verts, m16):
+    """FD::moveModel (moveModel.inl:6-30) on a copy of verts ((V, stride)
+    float32); m16: the column-major 4x4.  Returns the transformed copy."""
+    v = np.array(verts, np.float32, order="C", copy=True)
+    m = np.ascontiguousarray(np.asarray(m16, np.float32).ravel())
+    assert m.size == 16 and v.ndim == 2 and v.shape[1] >= 3
+    lib().or_move_model(v.ctypes.data, v.shape[1], v.shape[0], m.ctypes.data)
+    return v
 
 
 def shade(shadow, g0, g1, g2, g3, samples, cam, La, Ld, Ls):

@@ -897,7 +897,9 @@ extern "C" int fd_debug_read(fd_ctx* c, const char* name, void* dst, size_t maxb
         if (nbytes) e = hipMemcpy(out.data(), src, nbytes, hipMemcpyDeviceToHost);
     };
     const std::string n(name);
-    const bool pipelineRan = s.status == FD_EXIT_OK || s.status == FD_EXIT_NO_PAIRS || s.status == FD_EXIT_NO_VALID_TRI;
+    // the receiver tables exist once the receivers are sorted (frames that
+    // reach the pairing stage)
+    const bool pipelineRan = s.status == FD_EXIT_OK || s.status == FD_EXIT_NO_PAIRS || s.status == FD_EXIT_CAPACITY;
     if (n == "state") {
         out.resize(sizeof(s));
         std::memcpy(out.data(), &s, sizeof(s));

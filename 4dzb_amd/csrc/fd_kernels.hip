@@ -934,8 +934,19 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
     __shared__ int wsum[16];
     __shared__ int classCount[33], classBase[33];
     FdState* s = a.st;
-    if (s->status != FD_EXIT_OK) return;
+    const int status = s->status;
+    // the receivers' start / end table exists once they are sorted: FDLaunch
+    // binds the samples before it pairs the triangles (FD/api.cpp:505-522),
+    // so a frame that ends without pairs (or over the pair arena) has it too
+    if (status != FD_EXIT_OK && status != FD_EXIT_NO_PAIRS && status != FD_EXIT_CAPACITY) return;
     const int nb = s->nb;
+    if (status != FD_EXIT_OK) {
+        for (int k = threadIdx.x; k <= nb; k += blockDim.x) {
+            a.rxStart[k] = (int)a.histR[(size_t)k * a.nblkR];
+            a.rxEnd[k] = (k < nb) ? (int)a.histR[(size_t)(k + 1) * a.nblkR] : a.WH;
+        }
+        return;
+    }
     if (threadIdx.x < 33) classCount[threadIdx.x] = 0;
     __syncthreads();
     // bin tables and the cost class of every bin (0 = longest lists)

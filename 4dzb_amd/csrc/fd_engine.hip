@@ -210,7 +210,8 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
 
 namespace {
 
-int prepare(fd_ctx* c, FdArgs& a) {
+// The frame's arguments from the context's settings (no workspace yet).
+int frame_args(fd_ctx* c, FdArgs& a) {
     if (c->models.empty() || !c->pos || !c->nrm || !c->shadow || c->W <= 0 || c->H <= 0 || !c->lightSet)
         return FD_ERR_STATE;
     if (c->lightDesc.res_w != c->lightDesc.res_h || c->lightDesc.res_w < 1 || c->lightDesc.res_w > 64)
@@ -218,6 +219,12 @@ int prepare(fd_ctx* c, FdArgs& a) {
     fill_args(a, c->models[0], c->pos, c->nrm, c->W, c->H, c->light, c->eps, c->cull, c->gridLog2, c->tileLog2, c->shadow,
               c->masks, c->pairCap);
     a.occVariant = c->occVariant;
+    return FD_OK;
+}
+
+int prepare(fd_ctx* c, FdArgs& a) {
+    int rc = frame_args(c, a);
+    if (rc != FD_OK) return rc;
     FdArgs probe = a;
     const size_t bytes = layout(probe, nullptr, c->countOps);
     hipError_t e = c->arena.ensure(bytes);
@@ -279,6 +286,43 @@ hipError_t move_vertices(float* verts, int vstride, int nverts, const float m[16
 }
 }  // namespace fdh
 
+// ---------------------------------------------------------------------------
+// Allocation-free launch (fd_launch_async_noalloc).  The frame workspace lives
+// in this file-scope device array: every GPU the library runs on holds its own
+// copy until the process ends.  On the host the array's name is only a
+// stand-in, never a device address: fd_init resolves the current device's copy
+// once with hipGetSymbolAddress and keeps that address per device, and only
+// that address reaches kernels (inside FdArgs).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void* g_scratch[kMaxDevices] = {};   // device address of `scratch` per device (null: not resolved)
+
+// Resolve the current device's copy of the scratch array (once per device).
+int scratch_resolve(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return FD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (g_scratch[dev]) return FD_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_status(e);
+    if (!ptr) return FD_ERR_HIP;
+    g_scratch[dev] = ptr;
+    return FD_OK;
+}
+
+void* scratch_of(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    return g_scratch[dev];
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -290,6 +334,12 @@ int fd_init(fd_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FD_ERR_NO_DEVICE;
     fd_ctx* c = new fd_ctx();
     (void)hipGetDevice(&c->device);
+    // the allocation-free launch's scratch array on this device
+    int rc = scratch_resolve(c->device);
+    if (rc != FD_OK) {
+        delete c;
+        return rc;
+    }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return FD_ERR_HIP;
@@ -434,6 +484,37 @@ int fd_launch_async(fd_ctx* c, void* stream) {
     return hip_status(e);
 }
 
+int fd_launch_async_noalloc(fd_ctx* c, void* stream) {
+    if (!c) return FD_ERR_ARG;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return FD_ERR_HIP;
+    char* base = (char*)scratch_of(dev);
+    if (!base) return FD_ERR_STATE;   // no fd_init on this device
+    FdArgs a;
+    int rc = frame_args(c, a);
+    if (rc != FD_OK) return rc;
+    // the pair arena takes what the rest of the workspace leaves, up to the
+    // context's pair capacity (a frame with more pairs exits with
+    // FD_EXIT_CAPACITY, as with a small fd_set_pair_capacity)
+    FdArgs probe = a;
+    probe.pairCap = 0;
+    const size_t fixed = layout(probe, nullptr, false);
+    const size_t slack = 1024;   // alignment of the two pair arrays
+    if (fixed + slack + 8 > (size_t)FD_NOALLOC_SCRATCH_BYTES) {
+        std::fprintf(stderr, "fd4d: fd_launch_async_noalloc: the frame needs %zu bytes of workspace before its pair "
+                     "arena, the scratch holds %zu\n", fixed, (size_t)FD_NOALLOC_SCRATCH_BYTES);
+        return FD_ERR_SCRATCH;
+    }
+    a.pairCap = std::min<unsigned long long>(c->pairCap, (FD_NOALLOC_SCRATCH_BYTES - fixed - slack) / 8);
+    probe = a;
+    if (layout(probe, nullptr, false) > (size_t)FD_NOALLOC_SCRATCH_BYTES) return FD_ERR_SCRATCH;
+    layout(a, base, false);
+    hipError_t e = fdk::enqueue_frame(a, (hipStream_t)stream, nullptr);
+    c->last = a;
+    c->haveLast = true;
+    return hip_status(e);
+}
+
 int fd_get_stats(fd_ctx* c, void* stream, fd_stats* out) {
     if (!c || !c->haveLast) return FD_ERR_STATE;
     FdState s;
@@ -554,6 +635,7 @@ const char* fd_status_string(int code) {
         case FD_ERR_CAPACITY: return "pair arena too small";
         case FD_ERR_UNSUPPORTED_LIGHT: return "unsupported light grid";
         case FD_ERR_NO_DEVICE: return "no GPU device";
+        case FD_ERR_SCRATCH: return "frame workspace larger than the static scratch";
         default: return "unknown";
     }
 }

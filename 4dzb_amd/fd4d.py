@@ -23,6 +23,7 @@ FD_ERR_HIP = -3
 FD_ERR_CAPACITY = -4
 FD_ERR_UNSUPPORTED_LIGHT = -5
 FD_ERR_NO_DEVICE = -6
+FD_ERR_SCRATCH = -7
 
 EXIT_OK, EXIT_SAMPLE_MAXZ, EXIT_TRI_MAXZ, EXIT_NO_VALID_SAMPLE = 0, 1, 2, 3
 EXIT_EMPTY_AABB, EXIT_EMPTY_LP, EXIT_NO_VALID_TRI, EXIT_NO_PAIRS, EXIT_CAPACITY = 4, 5, 6, 7, 9
@@ -135,6 +136,7 @@ def lib():
             "fd_set_area_light": ([vp, ctypes.POINTER(AreaLight)], ip),
             "fd_launch": ([vp, vp, ctypes.POINTER(Stats)], ip),
             "fd_launch_async": ([vp, vp], ip),
+            "fd_launch_async_noalloc": ([vp, vp], ip),
             "fd_get_stats": ([vp, vp, ctypes.POINTER(Stats)], ip),
             "fd_set_pair_capacity": ([vp, sz], ip),
             "fd_move_model": ([vp, ctypes.POINTER(ctypes.c_float), vp], ip),

@@ -44,6 +44,7 @@ typedef struct fd_ctx fd_ctx;
 #define FD_ERR_CAPACITY (-4)          /* pair list larger than the pair arena */
 #define FD_ERR_UNSUPPORTED_LIGHT (-5) /* light grid not square or outside 1..64 */
 #define FD_ERR_NO_DEVICE (-6)         /* no GPU / kernels not loadable */
+#define FD_ERR_SCRATCH (-7)           /* workspace larger than FD_NOALLOC_SCRATCH_BYTES */
 
 /* Frame exit codes (fd_stats.exit_code), in the order of FD::FDLaunch's
  * early returns (4DZB/FDZB_Soft_Shadow/api.cpp:495-559). */
@@ -161,6 +162,23 @@ int fd_launch(fd_ctx* ctx, void* hip_stream, fd_stats* stats);
  * sequence is capturable into a HIP graph); fd_get_stats reads the result. */
 int fd_launch_async(fd_ctx* ctx, void* hip_stream);
 int fd_get_stats(fd_ctx* ctx, void* hip_stream, fd_stats* stats);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops or harnesses that
+ * forbid hipMalloc).  fd_launch_async with the frame workspace in a static
+ * device array of FD_NOALLOC_SCRATCH_BYTES (128 MiB) that each GPU holds until
+ * the process ends: calls no hipMalloc / hipFree / allocating routine.  The
+ * pair arena takes what the rest of the workspace leaves, up to the context's
+ * pair capacity (more pairs: the frame exits with FD_EXIT_CAPACITY).  A frame
+ * whose workspace does not fit returns FD_ERR_SCRATCH and launches nothing
+ * (1M receivers with 200k triangles and caller-provided masks fit; masks left
+ * NULL are placed in the scratch as well).  Needs fd_init on the current
+ * device; profiling (fd_set_profiling) does not apply.  Every call on one GPU
+ * shares that GPU's array: fd_get_stats / fd_debug_read read the last call's
+ * state only until the next call on that GPU, and calls that overlap on one
+ * GPU (other streams, threads or contexts) race. */
+#define FD_NOALLOC_SCRATCH_BYTES (128u << 20)
+int fd_launch_async_noalloc(fd_ctx* ctx, void* hip_stream);
 
 /* No counterpart (the reference's pair buffer is fixed at
  * FD_MAX_BT_PIAR_Num * 1.4 entries, FD/api.cpp:132-138): pair arena size. */

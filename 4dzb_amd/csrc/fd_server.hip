@@ -138,7 +138,11 @@ bool fd_server::try_move() {
         if (sl.busy)
             for (const Req& r : sl.reqs)
                 if (r.r.scene == q.moveScene) return false;
-    const Model& md = scenes[q.moveScene];
+    Model md;
+    {
+        std::lock_guard<std::mutex> lk(mu);   // fd_server_add_scene may grow scenes concurrently
+        md = scenes[q.moveScene];
+    }
     hipError_t e = move_vertices((float*)md.verts, md.vstride, md.nverts, q.m, moveStream);
     if (e == hipSuccess) e = hipStreamSynchronize(moveStream);
     if (e != hipSuccess) { error = FD_ERR_HIP; stop = true; cv.notify_all(); return false; }
@@ -290,11 +294,8 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (r->scene < 0 || r->scene >= (int)s->scenes.size()) return FD_ERR_ARG;
-        if (s->error.load()) return s->error.load();
-        q.id = s->nextId++;
-        md = s->scenes[r->scene];
+        md = s->scenes[r->scene];   // pointers and counts only: moves transform the buffers in place
     }
-    const long long id = q.id;
     q.r = *r;
     q.tSubmit = s->now();
     FdLight L;
@@ -317,7 +318,12 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     FdArgs probe = q.args;
     q.bytes = layout(probe, nullptr, false);
     if (q.bytes > s->pool.size()) return FD_ERR_CAPACITY;   // can never be admitted
+    // the id and the queue position are taken in one critical section, so
+    // that a concurrent move is ordered either before or after this request
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->error.load()) return s->error.load();
+    q.id = s->nextId++;
+    const long long id = q.id;
     s->incoming.push_back(std::move(q));
     ++s->submitted;
     return id;

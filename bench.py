@@ -2,20 +2,25 @@
 """Soft-shadow visibility benchmark on one MI355X.
 
 A "step" is one frame of the plain per-frame path (FD::FDLaunch's
-replacement) over the heightfield-1k workload: 1024x1024 receivers on
-z = 0.2 sin x cos y, a 2x2 area light at z = 6 sampled 32x32, and 200k
-occluder triangles (500 UV spheres).  Inputs are resident in HBM before the
-timed region; each frame writes the per-receiver shadow value and the
-1024-bit light-sample mask of every receiver.
+replacement) over the boxes-1080p workload, the largest frame that one GPU
+runs: 1920x1080 receivers from a ray-cast G-buffer (ground plane + 20
+boxes), a 3x1 area light at z = 6 sampled 64x64 (4096-bit masks), and 1M
+clustered occluder triangles.  Inputs are resident in HBM before the timed
+region; each frame writes the per-receiver shadow value and the 4096-bit
+light-sample mask of every receiver (1.06 GB).
 
 value = visibility Gbit/s = frames * W*H * Lw*Lh / seconds / 1e9.
 
-The roofline object describes the dominant kernel (the occlusion kernel):
-its duration comes from HIP events recorded around it on the launch stream,
-its work from the kernel's own counters (candidate tests, cull passes,
-visited light rows) priced with per-event operation counts of the
-reference's arithmetic.  cpu_baseline times the CPU restatement of the
-reference algorithm on this host (all cores) over a seeded receiver sample.
+The heightfield-1k frame (1024x1024 receivers, 32x32 light, 200k triangles)
+is measured the same way in its own object.  Each plain frame's roofline
+object describes the dominant kernel (the occlusion kernel): its duration
+comes from HIP events recorded around it on the launch stream, its work from
+the kernel's own counters (candidate tests, planes set up, light rows)
+priced with per-event operation counts of the reference's arithmetic, with
+the rows the reference visits (achieved) and the rows this kernel visits
+(achieved_kernel_rows).  cpu_baseline times the CPU restatement of the
+reference algorithm on this host, single-threaded and on all cores, over a
+seeded receiver sample.
 """
 import argparse
 import copy
@@ -42,14 +47,16 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
-PROFILED = {("heightfield-1k", 0): "k_occlude_heightfield-1k.md",
-            ("boxes-1080p", 32): "k_occlude_boxes-1080p-32x32.md"}
+PROFILED = {("heightfield-1k", 32): "k_occlude_heightfield-1k.md",
+            ("boxes-1080p", 32): "k_occlude_boxes-1080p-32x32.md",
+            ("boxes-1080p", 64): "k_occlude_boxes-1080p-64x64.md"}
 
 
 def profiled_traffic(workload, light_res):
-    """HBM bytes per occlusion launch from the committed rocprofv3 summary of
-    the same workload (FETCH_SIZE and WRITE_SIZE passes, calibrated with a
-    256 MiB copy), or None when no summary covers it."""
+    """HBM bytes per occlusion launch and the counter-derived VALU issue and
+    occupancy from the committed rocprofv3 summary of the same workload
+    (FETCH_SIZE and WRITE_SIZE passes, calibrated with a 256 MiB copy; SQ
+    counters), or None when no summary covers it."""
     name = PROFILED.get((workload, light_res))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
@@ -128,6 +135,75 @@ def gpu_plain(pkg, frame, steps, warmup, quick=False):
                 stats=st.as_dict(), shadow=shadow, masks=masks)
 
 
+def make_frame(wl, workload, light_res):
+    make = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[workload]
+    return make(light_res=light_res) if light_res else make()
+
+
+def occlusion_roofline(g, occ_ms, workload, light_res):
+    """Roofline object of the occlusion kernel: algorithmic operations per
+    launch from the kernel's counters (the reference's rows and, separately,
+    the rows this kernel visits) over the kernel's event-timed duration,
+    against the FP32 vector peak."""
+    cand, npass, rows, items, wave_it, wave_rows, ref_rows, planes = g["ops"]
+    ops_ref = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * ref_rows
+    ops_run = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * rows
+    achieved = ops_ref / (occ_ms * 1e-3) / 1e12
+    achieved_run = ops_run / (occ_ms * 1e-3) / 1e12
+    prof = profiled_traffic(workload, light_res)
+    return dict(bound="valu", achieved=round(achieved, 3), peak=FP32_VECTOR_PEAK_TFLOPS, unit="TOP/s",
+                frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
+                traffic=prof["traffic_bytes"] if prof else None,
+                achieved_kernel_rows=round(achieved_run, 3),
+                frac_kernel_rows=round(achieved_run / FP32_VECTOR_PEAK_TFLOPS, 4),
+                work_model=("achieved: 5 ops per candidate test + 111 per pair whose planes are set up + 15 per "
+                            "light row the reference visits (rows this kernel skips by its row window count); "
+                            "achieved_kernel_rows: the same with the rows this kernel visits"),
+                traffic_source=prof["source"] if prof else None,
+                valu_issue_profiled=prof.get("valu_issue") if prof else None,
+                occupancy_profiled=prof.get("occupancy") if prof else None,
+                kernel="k_occlude", kernel_ms=round(occ_ms, 4),
+                work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
+                          work_items=int(items), wave_tri_iterations=int(wave_it),
+                          wave_row_iterations=int(wave_rows),
+                          lane_util_rows=round(rows / max(1, 64 * wave_rows), 4),
+                          reference_row_visits=int(ref_rows), planes=int(planes),
+                          ops_per_event=[OPS_PER_CAND, OPS_PER_PLANES, OPS_PER_ROW]))
+
+
+def plain_leg(pkg, frame, workload, steps, warmup, quick):
+    """One plain frame configuration: throughput (frames back to back),
+    latency, stage times and the occlusion roofline."""
+    g = gpu_plain(pkg, frame, steps, warmup, quick)
+    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+    value = g["bits"] / (g["ms_per_step"] * 1e-3) / 1e9
+    occ_ms = max(g["stage"][4], 1e-9)
+    leg = dict(gbps=round(value, 4), ms_per_step=round(g["ms_per_step"], 4),
+               config=dict(workload=workload, receivers=f"{W}x{H}", light_samples=f"{R}x{R}",
+                           triangles=frame.scene.ntris, bins=f"{1 << frame.grid_log2}x{1 << frame.grid_log2}",
+                           tile_width=frame.tile_width, path="plain per-frame"),
+               p50_ms=round(pctl(g["lat"], 50), 3), p99_ms=round(pctl(g["lat"], 99), 3),
+               stage_ms=dict(zip(["light_plane", "receiver_binning", "triangle_pairing", "tables", "occlusion"],
+                                 [round(x, 4) for x in g["stage"]])),
+               frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
+                                valid_triangles=g["stats"]["valid_triangles"]),
+               roofline=occlusion_roofline(g, occ_ms, workload, R))
+    return g, leg
+
+
+def oracle_sample_check(frame, g, n):
+    """The GPU frame's masks and shadow values equal the oracle's on a seeded
+    receiver sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
+    ids = np.sort(np.random.default_rng(99).choice(W * H, n, replace=False))
+    o = oracle.run(frame, subset=ids)
+    wpr = 1 if R <= 32 else 2
+    return bool(np.array_equal(g["masks"].reshape(W * H, R * wpr)[ids], o.masks.reshape(W * H, R * wpr)[ids])
+                and np.array_equal(g["shadow"][ids].view(np.uint32), o.shadow[ids].view(np.uint32)))
+
+
 def gpu_batched(pkg, reps=3):
     """batch-64: 64 instances of uneven resolution / light grid / light pose
     over the heightfield-1k scene.  Plain = each instance through the
@@ -204,6 +280,40 @@ def gpu_batched(pkg, reps=3):
         t.append(time.perf_counter() - a)
     batched = min(t)
     b.close()
+    # per instance class (receiver size, light grid): the class's instances
+    # back to back through the plain path vs as one graph-replayed batch
+    classes = {}
+    for k, it in enumerate(items):
+        classes.setdefault((it["w"], it["h"], it["light"].res_w), []).append(k)
+    per_class = {}
+    for key, ks in sorted(classes.items()):
+        sub = [items[k] for k in ks]
+        tp = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            for it in sub:
+                ctx.set_receivers(it["pos"], it["nrm"], it["w"], it["h"])
+                ctx.set_outputs(it["shadow"], it["masks"])
+                ctx.set_area_light(it["light"])
+                ctx.launch_async(strm.handle)
+            strm.sync()
+            tp.append(time.perf_counter() - a)
+        bc = fd.Batch(ctx, sub)
+        bc.capture(strm.handle)
+        bc.launch(strm.handle)
+        strm.sync()
+        tb = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            bc.launch(strm.handle)
+            strm.sync()
+            tb.append(time.perf_counter() - a)
+        bc.close()
+        cb = sum(it["w"] * it["h"] * it["light"].res_w ** 2 for it in sub) / 1e9
+        per_class[f"{key[0]}x{key[1]}:{key[2]}"] = dict(
+            instances=len(sub), plain_ms=round(min(tp) * 1e3, 3), batched_ms=round(min(tb) * 1e3, 3),
+            plain_gbps=round(cb / min(tp), 2), batched_gbps=round(cb / min(tb), 2),
+            speedup=round(min(tp) / min(tb), 3))
     ctx.close()
     gb = bits / 1e9
     return dict(workload="batch-64 (heightfield-1k scene, 200k triangles; receivers from {256^2, 512^2, 1024^2, "
@@ -214,6 +324,7 @@ def gpu_batched(pkg, reps=3):
                 plain_sync_gbps=round(gb / plain_sync, 3), plain_async_gbps=round(gb / plain_async, 3),
                 batched_gbps=round(gb / batched, 3), speedup_vs_plain_sync=round(plain_sync / batched, 3),
                 speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact),
+                per_class=per_class,
                 autotune=dict(seconds=round(t_tune, 2), classes={k: dict(bins=v["bins"], tile=v["tile"],
                                                                         ms=v["ms"], default_ms=v["default_ms"])
                                                                for k, v in sorted(cache.items())}))
@@ -292,44 +403,68 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
         t_free = start + sv
         lat_plain.append(t_free - a)
     plain_span = t_free - arr[0]
-    # served
-    # same pair arena per request as the plain path's contexts (64M pairs)
-    srv = fd.Server(budget_bytes=budget_gb << 30, streams=4, max_batch=16, eps=1e-3, max_pairs=64 << 20,
-                    tune=at.server_table(cache))
-    for (v, i), sc in zip(dev_scenes, scenes):
-        srv.add_scene(v, sc.nverts, 16, i, sc.ntris)
-    for o in outs:
-        o[0].upload(np.zeros(o[0].nbytes, np.uint8))
-        o[1].upload(np.zeros(o[1].nbytes, np.uint8))
-    t0 = srv.now_ms()
-    for k, q in enumerate(reqs):
-        target = t0 + arr[k]
-        while srv.now_ms() < target:
-            dt = (target - srv.now_ms()) / 1e3
-            if dt > 0.0005:
-                time.sleep(dt - 0.0003)
-        r, pos, nrm = rx[(q["w"], q["h"])]
-        srv.submit(q["scene"], pos, nrm, q["w"], q["h"], lights[k], outs[k][0], outs[k][1], tag=k)
-    srv.drain(600000)
-    done = sorted(srv.poll(len(reqs) + 8), key=lambda c: c.user_tag)
-    srv.close()
-    lat = [c.t_done_ms - c.t_submit_ms for c in done]
-    span = max(c.t_done_ms for c in done) - min(c.t_submit_ms for c in done)
-    failed = sum(1 for c in done if c.status != 0)
-    mismatched = sum(1 for k in range(len(reqs)) if _digest(outs[k]) != ref[k])
-    exact = failed == 0 and mismatched == 0 and len(done) == len(reqs)
+    # served: the continuous-batching server at the given arrival times
+    def serve(arrivals):
+        # same pair arena per request as the plain path's contexts (64M pairs)
+        srv = fd.Server(budget_bytes=budget_gb << 30, streams=4, max_batch=16, eps=1e-3, max_pairs=64 << 20,
+                        tune=at.server_table(cache))
+        for (v, i), sc in zip(dev_scenes, scenes):
+            srv.add_scene(v, sc.nverts, 16, i, sc.ntris)
+        for o in outs:
+            o[0].upload(np.zeros(o[0].nbytes, np.uint8))
+            o[1].upload(np.zeros(o[1].nbytes, np.uint8))
+        t0 = srv.now_ms()
+        for k, q in enumerate(reqs):
+            target = t0 + arrivals[k]
+            while srv.now_ms() < target:
+                dt = (target - srv.now_ms()) / 1e3
+                if dt > 0.0005:
+                    time.sleep(dt - 0.0003)
+            r, pos, nrm = rx[(q["w"], q["h"])]
+            srv.submit(q["scene"], pos, nrm, q["w"], q["h"], lights[k], outs[k][0], outs[k][1], tag=k)
+        srv.drain(600000)
+        done = sorted(srv.poll(len(reqs) + 8), key=lambda c: c.user_tag)
+        srv.close()
+        lat = [c.t_done_ms - c.t_submit_ms for c in done]
+        span = max(c.t_done_ms for c in done) - min(c.t_submit_ms for c in done)
+        failed = sum(1 for c in done if c.status != 0)
+        mismatched = sum(1 for k in range(len(reqs)) if _digest(outs[k]) != ref[k])
+        return dict(lat=lat, span=span, failed=failed, mismatched=mismatched,
+                    exact=failed == 0 and mismatched == 0 and len(done) == len(reqs),
+                    mean_batch=float(np.mean([c.batch_size for c in done])))
+
+    over = serve([a - arr[0] for a in arr])
     gb = bits / 1e9
+    # a sustainable load: 60% of the capacity measured above, same requests
+    cap_rps = len(reqs) / (over["span"] * 1e-3)
+    rate2 = 0.6 * cap_rps
+    arr2 = [(a - arr[0]) * rate / rate2 for a in arr]
+    sus = serve(arr2)
+    t_free, lat_plain2 = 0.0, []
+    for a, sv in zip(arr2, svc):
+        start = max(a, t_free)
+        t_free = start + sv
+        lat_plain2.append(t_free - a)
     return dict(workload=f"serve-poisson ({nreq} requests at {rate:.0f} req/s; scenes 100k/300k/1M/2M triangles; "
                 "receivers from {256^2, 512^2, 1024^2, 1920x1080}; light grids 16^2 / 32^2)",
                 requests=len(reqs), gbit=round(gb, 3), offered_span_ms=round(arr[-1] - arr[0], 1),
-                served_span_ms=round(span, 1), plain_span_ms=round(plain_span, 1),
-                served_gbps=round(gb / (span * 1e-3), 3), plain_gbps=round(gb / (plain_span * 1e-3), 3),
-                speedup=round(plain_span / span, 3),
-                served_p50_ms=round(pctl(lat, 50), 2), served_p99_ms=round(pctl(lat, 99), 2),
+                served_span_ms=round(over["span"], 1), plain_span_ms=round(plain_span, 1),
+                served_gbps=round(gb / (over["span"] * 1e-3), 3), plain_gbps=round(gb / (plain_span * 1e-3), 3),
+                speedup=round(plain_span / over["span"], 3),
+                served_p50_ms=round(pctl(over["lat"], 50), 2), served_p99_ms=round(pctl(over["lat"], 99), 2),
                 plain_p50_ms=round(pctl(lat_plain, 50), 2), plain_p99_ms=round(pctl(lat_plain, 99), 2),
                 plain_frame_ms_mean=round(float(np.mean(svc)), 2),
-                mean_batch=round(float(np.mean([c.batch_size for c in done])), 2),
-                masks_bit_exact_vs_plain=bool(exact), failed_requests=failed, mismatched_requests=mismatched,
+                mean_batch=round(over["mean_batch"], 2),
+                masks_bit_exact_vs_plain=bool(over["exact"] and sus["exact"]), failed_requests=over["failed"],
+                mismatched_requests=over["mismatched"],
+                sustainable=dict(note="the same requests at 60% of the capacity measured at 500 req/s",
+                                 rate_rps=round(rate2, 1), capacity_rps=round(cap_rps, 1),
+                                 served_p50_ms=round(pctl(sus["lat"], 50), 2),
+                                 served_p99_ms=round(pctl(sus["lat"], 99), 2),
+                                 plain_p50_ms=round(pctl(lat_plain2, 50), 2),
+                                 plain_p99_ms=round(pctl(lat_plain2, 99), 2),
+                                 served_span_ms=round(sus["span"], 1), mean_batch=round(sus["mean_batch"], 2),
+                                 failed_requests=sus["failed"], mismatched_requests=sus["mismatched"]),
                 autotune=dict(seconds=round(t_tune, 2), classes=len(cache),
                               non_default=sum(1 for v in cache.values() if (v["bins"], v["tile"]) != (16, 8))))
 
@@ -548,18 +683,23 @@ def gpu_app_pipeline(pkg, frames=30):
                 note="stage times include the host round trip of one call (launch + stream sync)")
 
 
-def cpu_baseline(frame, gpu, seconds):
-    """The oracle (CPU restatement of the reference) on a receiver sample,
-    all host cores; full-frame time = fixed stages + occlusion scaled."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_run(oracle, frame, gpu, seconds, nthreads, rng):
+    """One timed oracle run over a receiver sample sized to about `seconds`
+    of occlusion work; returns the full-frame estimate."""
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
-    nthreads = oracle.max_threads()
-    rng = np.random.default_rng(123)
-    n = 4096
+    n = 2048 if nthreads == 1 else 8192
     o = oracle.run(frame, subset=rng.choice(W * H, n, replace=False), nthreads=nthreads)
     per_rx = o.scalars["ms_occ"] / max(1, int(o.computed.sum()))
-    # grow the sample to fill about `seconds` of occlusion work
     n2 = int(min(W * H, max(n, seconds * 1e3 / max(per_rx, 1e-6))))
     sub = rng.choice(W * H, n2, replace=False)
     t0 = time.perf_counter()
@@ -571,17 +711,35 @@ def cpu_baseline(frame, gpu, seconds):
     binned = int((o.key < int(s["num_bins"])).sum())
     occ_full_ms = s["ms_occ"] * binned / max(1, computed)
     frame_ms = fixed_ms + occ_full_ms
-    value = W * H * R * R / (frame_ms * 1e-3) / 1e9
     ids = sub[o.computed[sub] > 0]
     wpr = 1 if R <= 32 else 2
     match = bool(np.array_equal(gpu["masks"].reshape(W * H, R * wpr)[ids], o.masks.reshape(W * H, R * wpr)[ids])
                  and np.array_equal(gpu["shadow"][ids].view(np.uint32), o.shadow[ids].view(np.uint32)))
-    return dict(value=round(value, 6), unit="Gbit/s", cores=nthreads, kind="port",
-                sample=(f"oracle on {W}x{H} receivers / {frame.scene.ntris} triangles: all stages in full, "
-                        f"occlusion of {computed} sampled receivers ({wall:.1f} s wall), frame time = fixed "
-                        f"stages {fixed_ms:.0f} ms + occlusion scaled to {binned} binned receivers "
-                        f"({occ_full_ms:.0f} ms)"),
-                frame_ms=round(frame_ms, 2), masks_match_gpu_on_sample=match, sample_receivers=computed)
+    return dict(value=round(W * H * R * R / (frame_ms * 1e-3) / 1e9, 6), frame_ms=round(frame_ms, 2),
+                sample=(f"oracle on {W}x{H} receivers / {frame.scene.ntris} triangles, {nthreads} thread(s): all "
+                        f"stages in full ({fixed_ms:.0f} ms), occlusion of {computed} sampled receivers "
+                        f"({wall:.1f} s wall), scaled to the {binned} binned receivers ({occ_full_ms:.0f} ms)"),
+                masks_match_gpu_on_sample=match, sample_receivers=computed)
+
+
+def cpu_baseline(frame, gpu, seconds):
+    """The oracle (CPU restatement of the reference, compiled -O3
+    -ffp-contract=off) on this host: one thread and all cores, each over a
+    receiver sample; full-frame time = fixed stages + occlusion scaled."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nthreads = oracle.max_threads()
+    rng = np.random.default_rng(123)
+    multi = _cpu_run(oracle, frame, gpu, seconds, nthreads, rng)
+    single = _cpu_run(oracle, frame, gpu, seconds, 1, rng)
+    return dict(value=multi["value"], unit="Gbit/s", cores=nthreads, kind="port", sample=multi["sample"],
+                cpu_model=cpu_model(), frame_ms=multi["frame_ms"],
+                masks_match_gpu_on_sample=multi["masks_match_gpu_on_sample"] and single["masks_match_gpu_on_sample"],
+                sample_receivers=multi["sample_receivers"],
+                single_thread=dict(value=single["value"], unit="Gbit/s", cores=1, frame_ms=single["frame_ms"],
+                                   sample=single["sample"]),
+                gpu_vs_cpu=dict(all_cores=round(gpu["gbps"] / max(multi["value"], 1e-12), 1),
+                                one_core=round(gpu["gbps"] / max(single["value"], 1e-12), 1)))
 
 
 def main():
@@ -589,8 +747,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="heightfield-1k", choices=["heightfield-1k", "plane-256", "boxes-1080p"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--workload", default="boxes-1080p", choices=["heightfield-1k", "plane-256", "boxes-1080p"])
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="occlusion seconds per CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="timed frames only (for profiler runs)")
     ap.add_argument("--no-batch", action="store_true")
@@ -618,50 +776,32 @@ def main():
         print(json.dumps({args.leg: legs[args.leg]()}), flush=True)
         return
     log(f"[bench] generating {args.workload}")
-    make = {"heightfield-1k": wl.heightfield_1k, "plane-256": wl.plane_256, "boxes-1080p": wl.boxes_1080p}[
-        args.workload]
-    frame = make(light_res=args.light_res) if args.light_res else make()
+    frame = make_frame(wl, args.workload, args.light_res)
     if args.bins:
         frame.grid_log2 = args.bins.bit_length() - 1
     if args.tile:
         frame.tile_width = args.tile
     log("[bench] GPU plain path")
-    g = gpu_plain(pkg, frame, args.steps, args.warmup, args.quick)
-    W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
-    value = g["bits"] / (g["ms_per_step"] * 1e-3) / 1e9
-    occ_ms = max(g["stage"][4], 1e-9)
-    cand, npass, rows, items, wave_it, wave_rows, ref_rows, planes = g["ops"]
-    ops = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * ref_rows
-    achieved = ops / (occ_ms * 1e-3) / 1e12
-    prof = profiled_traffic(args.workload, args.light_res)
-    roofline = dict(bound="valu", achieved=round(achieved, 3), peak=FP32_VECTOR_PEAK_TFLOPS, unit="TOP/s",
-                    frac=round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
-                    traffic=prof["traffic_bytes"] if prof else None,
-                    traffic_source=prof["source"] if prof else None,
-                    valu_issue_profiled=prof.get("valu_issue") if prof else None,
-                    kernel="k_occlude", kernel_ms=round(occ_ms, 4),
-                    work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
-                              work_items=int(items), wave_tri_iterations=int(wave_it),
-                              wave_row_iterations=int(wave_rows),
-                              lane_util_rows=round(rows / max(1, 64 * wave_rows), 4),
-                              reference_row_visits=int(ref_rows), planes=int(planes),
-                              ops_per_event=[OPS_PER_CAND, OPS_PER_PLANES, OPS_PER_ROW]))
-    out = dict(metric=METRIC, value=round(value, 4), unit="Gbit/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
-               ms_per_step=round(g["ms_per_step"], 4), higher_is_better=True, scaling="weak", vs_baseline=None,
+    g, leg = plain_leg(pkg, frame, args.workload, args.steps, args.warmup, args.quick)
+    out = dict(metric=METRIC, value=leg["gbps"], unit="Gbit/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=leg["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f32", data="synthetic (seeded generators, 4dzb_amd/csrc/scenes.cpp)",
-               config=dict(workload=args.workload, receivers=f"{W}x{H}", light_samples=f"{R}x{R}",
-                           triangles=frame.scene.ntris, bins=f"{1 << frame.grid_log2}x{1 << frame.grid_log2}",
-                           tile_width=frame.tile_width, path="plain per-frame"),
-               p50_ms=round(pctl(g["lat"], 50), 3), p99_ms=round(pctl(g["lat"], 99), 3),
-               stage_ms=dict(zip(["light_plane", "receiver_binning", "triangle_pairing", "tables", "occlusion"],
-                                 [round(x, 4) for x in g["stage"]])),
-               frame_stats=dict(valid_receivers=g["stats"]["valid_receivers"], pairs=g["stats"]["pairs"],
-                                valid_triangles=g["stats"]["valid_triangles"]),
-               roofline=roofline)
+               config=leg["config"], p50_ms=leg["p50_ms"], p99_ms=leg["p99_ms"], stage_ms=leg["stage_ms"],
+               frame_stats=leg["frame_stats"], roofline=leg["roofline"])
+    if not args.quick and args.workload != "heightfield-1k" and not args.light_res and not args.bins \
+            and not args.tile:
+        log("[bench] GPU plain path, heightfield-1k")
+        f2 = make_frame(wl, "heightfield-1k", 0)
+        g2, leg2 = plain_leg(pkg, f2, "heightfield-1k", args.steps, args.warmup, False)
+        leg2["masks_match_oracle_on_sample"] = oracle_sample_check(f2, g2, 3000)
+        out["plain_heightfield_1k"] = leg2
+        log("[bench] bin-queue occlusion variant (heightfield-1k)")
+        out["bin_queue_variant"] = dict(gpu_bin_queue(pkg, g2, f2), workload="heightfield-1k")
+        g2["run"].close()
     if not args.quick and not args.bins and not args.tile:
         log("[bench] autotuned layout (same frame)")
         out["autotuned_layout"] = gpu_autotuned_layout(pkg, frame)
-    if not args.quick:
+    if not args.quick and args.workload == "heightfield-1k":
         log("[bench] bin-queue occlusion variant (same frame)")
         out["bin_queue_variant"] = gpu_bin_queue(pkg, g, frame)
     if not args.no_batch and not args.quick:
@@ -678,6 +818,7 @@ def main():
         out["shard_tile_1_of_8"] = gpu_shard_tile(pkg)
     if not args.no_cpu and not args.quick:
         log("[bench] CPU baseline (oracle)")
+        g["gbps"] = out["value"]
         out["cpu_baseline"] = cpu_baseline(frame, g, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -684,7 +684,6 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
         hArgs[i] = args[i];
         if (slots > 2048) hArgs[i].segs = 1;
         hArgs[i].binLds = P.maxNb <= FD_LDS_BINS;   // the batched launch sizes LDS for the largest grid
-        hArgs[i].rowParts = hArgs[i].L.wpr == 2 ? FD_ROW_PARTS : 1;   // 64-bit rows: split work items
     }
     int k = n;
     for (int cl = 0; cl < 2; ++cl)
@@ -723,8 +722,8 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
                 case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
                 case FD_MAP_SCANO: blocks = fdk::scan_chunks(std::max(a.ntris, 1)); break;
                 case FD_MAP_ZERO: blocks = a.segs > 1 ? fdk::zero_blocks(a) : 0; break;
-                case FD_MAP_FIN: blocks = (a.segs > 1 || a.rowParts > 1) ? a.chunkSlots : 0; break;
-                default: blocks = a.chunkSlots * a.segs * std::max(a.rowParts, 1); break;
+                case FD_MAP_FIN: blocks = a.segs > 1 ? a.chunkSlots : 0; break;
+                default: blocks = a.chunkSlots * a.segs; break;
             }
             acc += blocks;
             mp[q + 1] = acc;

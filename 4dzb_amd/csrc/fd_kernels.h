@@ -69,9 +69,6 @@ struct FdArgs {
     int binLds;
     // occlusion kernel: FD_OCC_PER_SAMPLE (the contract) or FD_OCC_BIN_QUEUE
     int occVariant;
-    // per-sample occlusion of 64-bit rows: work items per chunk, each taking
-    // 64 / rowParts rows (1 = all rows in one work item); set at launch
-    int rowParts;
 };
 
 // Waves per occlusion work item (per-sample kernel), for 32-bit and 64-bit
@@ -84,12 +81,6 @@ struct FdArgs {
 #define FD_OCC_WAVES64 8
 #endif
 
-// Row parts of a 64-bit-row work item (each part a work item of 64 / parts
-// rows, repeating the plane set-up).  With FD_OCC_WAVES64 waves sharing a
-// work item's LDS, whole rows (1 part) are fastest.
-#ifndef FD_ROW_PARTS
-#define FD_ROW_PARTS 1
-#endif
 
 #define FD_LDS_BINS 1024
 #define FD_BIN_LDS_BYTES 20

@@ -1332,27 +1332,25 @@ __device__ __forceinline__ bool point_in_tri(const float* A, const float* B, con
     return true;
 }
 
-// Rows rowBase .. rowBase + rows - 1 of the work item's receivers go out;
-// LDS holds them from local row 0, the high words ldsRows rows further.
+// The work item's rows go out; LDS holds row r's low words at [r][lane],
+// the high words (64-bit rows) R rows further.  The waves of the work item
+// share the write-out.
 __device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr,
-                                            int rowBase, int rows, int ldsRows, int wave = 0, int waves = 1) {
+                                            int wave = 0, int waves = 1) {
     // coalesced write-out: consecutive lanes write consecutive words of one
     // receiver's mask (rows x wpr words); the receiver id comes from its
     // lane by a cross-lane read with every lane active
-    const int per = R * wpr, perH = rows * wpr;
-    const int total = n * perH;
+    const int per = R * wpr;
+    const int total = n * per;
     for (int j0 = wave * 64; j0 < total; j0 += 64 * waves) {
         const int j = j0 + lane_id();
-        const int l = min(j, total - 1) / perH, w = j - l * perH;
+        const int l = min(j, total - 1) / per, w = j - l * per;
         const int idl = __shfl(id, l, 64);
         if (j < total) {
             const int r = w / wpr, h = w - r * wpr;
-            a.masks[(size_t)idl * per + rowBase * wpr + w] = lds[(h * ldsRows + r) * 64 + l];
+            a.masks[(size_t)idl * per + w] = lds[(h * R + r) * 64 + l];
         }
     }
-}
-__device__ __forceinline__ void store_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
-    store_masks(a, lds, id, n, R, wpr, 0, R, R);
 }
 
 // Masks of n receivers (lane l holds the id of receiver l) set to all
@@ -1374,22 +1372,19 @@ __device__ __forceinline__ void fill_masks(const FdArgs& a, int id, bool back, i
 // Merge of a segment's partial masks: atomic OR of the non-zero words into
 // the (zeroed) output; same word order as store_masks.
 __device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr,
-                                         int rowBase, int rows, int ldsRows, int wave = 0, int waves = 1) {
-    const int per = R * wpr, perH = rows * wpr;
-    const int total = n * perH;
+                                         int wave = 0, int waves = 1) {
+    const int per = R * wpr;
+    const int total = n * per;
     for (int j0 = wave * 64; j0 < total; j0 += 64 * waves) {
         const int j = j0 + lane_id();
-        const int l = min(j, total - 1) / perH, w = j - l * perH;
+        const int l = min(j, total - 1) / per, w = j - l * per;
         const int idl = __shfl(id, l, 64);
         if (j < total) {
             const int r = w / wpr, h = w - r * wpr;
-            const uint32_t v = lds[(h * ldsRows + r) * 64 + l];
-            if (v) atomicOr(a.masks + (size_t)idl * per + rowBase * wpr + w, v);
+            const uint32_t v = lds[(h * R + r) * 64 + l];
+            if (v) atomicOr(a.masks + (size_t)idl * per + w, v);
         }
     }
-}
-__device__ __forceinline__ void or_masks(const FdArgs& a, const uint32_t* lds, int id, int n, int R, int wpr) {
-    or_masks(a, lds, id, n, R, wpr, 0, R, R);
 }
 
 // Work item -> rank of its bin in the occlusion order: binary search over
@@ -1587,24 +1582,14 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const int lane = lane_id();
     const int wv = NW > 1 ? uniform((int)(threadIdx.x >> 6)) : 0;   // wave of the work item
     const T FULL = (R >= Bits<T>::width) ? ~(T)0 : (((T)1 << R) - (T)1);
-    // work item = (row part, triangle segment seg, chunk c); segment-major
-    // so that neighbouring work items share a stretch of the same triangle
-    // list.  64-bit rows are split into NH parts of 64 / NH rows (NH work
-    // items per chunk): less LDS per wave, more waves per CU.
+    // work item = (triangle segment seg, chunk c); segment-major so that
+    // neighbouring work items share a stretch of the same triangle list
     const int S = a.segs;
-    const int NH = (Bits<T>::width == 64 && a.rowParts > 1) ? a.rowParts : 1;
-    const int part = blkIn / (a.chunkSlots * S);
-    const int blkS = blkIn - part * a.chunkSlots * S;
-    const int seg = blkS / a.chunkSlots;
-    const int c = blkS - seg * a.chunkSlots;
-    const int ldsRows = NH > 1 ? 64 / NH : R;             // rows held in LDS (per 32-bit word plane)
-    const int rowBase = part * ldsRows;
-    const int rowsH = NH > 1 ? min(ldsRows, R - rowBase) : R;   // rows of this work item
-    if (rowsH <= 0) return;                                // (a part beyond a grid of fewer than 64 rows)
-    const T partMask = NH > 1 ? (T)((((1ull << (rowsH - 1)) << 1) - 1ull) << (rowBase & 63)) : FULL;
+    const int seg = blkIn / a.chunkSlots;
+    const int c = blkIn - seg * a.chunkSlots;
 
     if (status != FD_EXIT_OK) {
-        if (seg != 0 || part != 0 || wv != 0) return;
+        if (seg != 0 || wv != 0) return;
         // early exit of the frame: every receiver shadow 0, except back-facing
         // receivers once the back-face stage has run (FD/api.cpp:489,
         // Step2_1_samplerRectangleAABB.inl:97)
@@ -1624,7 +1609,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     const int nb = s->nb;
     const int rank = chunk_rank(a, c, nb);
     const int bin = uniform(a.binAtRank[rank]);
-    if (bin == nb && (seg != 0 || part != 0)) return;
+    if (bin == nb && seg != 0) return;
     // this work item's stretch of the bin's triangle list (bins 0..nb-1)
     int ts = 0, te = 0;
     if (bin != nb) {
@@ -1659,9 +1644,9 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 
     FdRxRec q{};
     if (act) q = a.sortedRec[s0 + lane];
-    for (int r = wv; r < ldsRows; r += NW) {
+    for (int r = wv; r < R; r += NW) {
         lds[r * 64 + lane] = 0u;
-        if (wpr == 2) lds[(ldsRows + r) * 64 + lane] = 0u;
+        if (wpr == 2) lds[(R + r) * 64 + lane] = 0u;
     }
     if (NW > 1) __syncthreads();
     if (CTA) {
@@ -1714,7 +1699,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     for (int ti = ts + wv; ti < te; ti += NW) {
         const uint64_t liveMask = __ballot(live);
         if (liveMask == 0) break;
-        if (COUNT && part == 0) { nCand += __popcll(liveMask); ++nWaveIt; }
+        if (COUNT) { nCand += __popcll(liveMask); ++nWaveIt; }
         const int tid = triList[ti];
         if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
@@ -1722,7 +1707,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         const bool pass = live && t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx &&
                           q.rmaxy <= t.bmaxy && t.minz <= q.z;
         if (!pass) continue;
-        if (COUNT && part == 0) ++nPass;
+        if (COUNT) ++nPass;
         // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172) start from
         // a_k = v_{k+1} - Q; a_2 = v0 - Q also gives
         // dir = dot(Q - v0, n) (:130-132) = -dot(v0 - Q, n), exactly (every
@@ -1786,7 +1771,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         const float mAB = fmaxf(fmaxf(fmaxf(fabsf(e[0].A), fabsf(e[1].A)), fabsf(e[2].A)),
                                 fmaxf(fmaxf(fabsf(e[0].B), fabsf(e[1].B)), fabsf(e[2].B)));
         const bool big = !(mAB <= 1.2676506e+30f);
-        if (COUNT && part == 0) {
+        if (COUNT) {
             // the rows the reference visits for this pair: its row-range mask
             // minus the rows already full (general form, before any window)
             ++nPlanes;
@@ -1813,7 +1798,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) D[k] = e[k].A * fr + e[k].C;
                 const T M = rowMask(D);
-                uint32_t* w0 = lds + (r - rowBase) * 64 + lane;
+                uint32_t* w0 = lds + r * 64 + lane;
                 T cur;
                 if (Bits<T>::width == 32) {
                     if (NW > 1) {
@@ -1823,7 +1808,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                         *w0 = (uint32_t)cur;
                     }
                 } else {
-                    uint32_t* w1 = lds + (ldsRows + r - rowBase) * 64 + lane;
+                    uint32_t* w1 = lds + (R + r) * 64 + lane;
                     const uint64_t mm = (uint64_t)M;
                     uint32_t lo32, hi32;
                     if (NW > 1) {
@@ -1867,7 +1852,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             // rows that may hold occluded samples (:176-191), then the row
             // window; a wave none of whose lanes keeps a row skips the
             // triangle at either step (the cheaper test first)
-            T valid = (~full) & partMask & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
+            T valid = (~full) & FULL & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
             if (!__any(valid != (T)0)) continue;
             valid &= row_window<T>(e, Rf, mAB);
             if (!__any(valid != (T)0)) continue;   // no lane can gain a bit from this triangle
@@ -1893,11 +1878,11 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                 e[k].rB = rcp_or_nan(e[k].B);
                 DN[k] = (e[k].B > 0 ? -e[k].nColN : 0.0f) + e[k].C;
             }
-            const T valid = (~full) & partMask & three_edges<T, true>(e, DN, Rf, R);
+            const T valid = (~full) & FULL & three_edges<T, true>(e, DN, Rf, R);
             wave_rows(valid);
             rows(valid, [&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
         }
-        if (full == partMask) live = false;
+        if (full == FULL) live = false;
     }
     if (COUNT) {
         // per-lane counts: nCand is wave-wide already
@@ -1922,9 +1907,9 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
     }
     __syncthreads();
-    // fraction = popcount / (W*H) (:226-231); split lists or rows: after the
-    // merge.  The write-out is shared by the waves.
-    if (wv == 0 && S == 1 && NH == 1) {
+    // fraction = popcount / (W*H) (:226-231); split lists: after the merge.
+    // The write-out is shared by the waves.
+    if (wv == 0 && S == 1) {
         int cnt = 0;
         for (int r = 0; r < R; ++r) {
             cnt += __popc(lds[r * 64 + lane]);
@@ -1932,8 +1917,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         }
         if (act) a.shadow[id] = (float)cnt / (float)a.L.resNum;
     }
-    if (S == 1) store_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows, wv, NW);
-    else or_masks(a, lds, id, n, R, wpr, rowBase, rowsH, ldsRows, wv, NW);
+    if (S == 1) store_masks(a, lds, id, n, R, wpr, wv, NW);
+    else or_masks(a, lds, id, n, R, wpr, wv, NW);
 }
 
 // Split lists only: zeroes the mask output before the occlusion merges into
@@ -1946,7 +1931,7 @@ __device__ __forceinline__ void zero_masks_body(const FdArgs& a, const int blk) 
         if (base + k < n) a.masks[base + k] = 0u;
 }
 
-// Split lists or split 64-bit rows: shadow value of every receiver of a
+// Split lists: shadow value of every receiver of a
 // real bin from its merged mask (popcount / samples, :226-231).
 __device__ __forceinline__ void occlude_finish_body(const FdArgs& a, const int c) {
     __shared__ int cnt[64];
@@ -2282,8 +2267,7 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
     for (int c = 0; c < 2; ++c) {
         if (p.nocc[c] == 0) continue;
-        // 64-bit rows run in FD_ROW_PARTS parts of 64 / FD_ROW_PARTS rows
-        const size_t shm = (size_t)(c + 1) * (c == 1 ? 64 / FD_ROW_PARTS : p.maxRes[c]) * 64 * sizeof(uint32_t);
+        const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t);
         if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64 * fd_occ_waves<uint32_t>(), shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
         else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64 * fd_occ_waves<uint64_t>(), shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
     }
@@ -2373,11 +2357,9 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         if (wpr == 1) k_occlude_cta<uint32_t><<<chunks, 64, shmCta, st>>>(a);
         else k_occlude_cta<uint64_t><<<chunks, 64, shmCta, st>>>(a);
     } else {
-        // 64-bit rows: FD_ROW_PARTS work items per chunk, 64 / FD_ROW_PARTS rows each
-        FdArgs ao = a;
-        ao.rowParts = wpr == 2 ? FD_ROW_PARTS : 1;
-        const int chunks = a.chunkSlots * a.segs * ao.rowParts;
-        const size_t shm = (size_t)wpr * (wpr == 2 ? 64 / FD_ROW_PARTS : a.L.res) * 64 * sizeof(uint32_t);
+        const FdArgs& ao = a;
+        const int chunks = a.chunkSlots * a.segs;
+        const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t);
         const int threads = 64 * (wpr == 2 ? fd_occ_waves<uint64_t>() : fd_occ_waves<uint32_t>());
         if (a.opCount) {
             if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64 * fd_occ_waves<uint32_t, true>(), shm, st>>>(ao);
@@ -2387,7 +2369,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
             else k_occlude<uint64_t, false><<<chunks, threads, shm, st>>>(ao);
         }
     }
-    if (a.segs > 1 || (wpr == 2 && FD_ROW_PARTS > 1 && a.occVariant != FD_OCC_BIN_QUEUE)) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
+    if (a.segs > 1) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);
     mark(5);
     return hipGetLastError();
 }

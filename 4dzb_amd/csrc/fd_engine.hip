@@ -91,6 +91,7 @@ struct fd_ctx {
     int gridLog2 = 4;
     int tileLog2 = 3;     // 8x8 receiver tiles
     int occVariant = FD_OCC_PER_SAMPLE;
+    int triOrder = FD_TRI_ORDER_AUTO;
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
     std::vector<Model> models;
@@ -187,6 +188,7 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     a.triRange = (uint2*)take(T * sizeof(uint2));
     a.histT = (uint32_t*)take(a.nT * sizeof(uint32_t));
     a.triOff = (uint32_t*)take(T * sizeof(uint32_t));
+    a.triClass = (uint8_t*)take(T);
     a.pairKT = (uint32_t*)take((size_t)a.pairCap * sizeof(uint32_t));
     a.sortedTri = (int*)take((size_t)a.pairCap * sizeof(int));
     int* tb = (int*)take((size_t)(6 * (nb + 2)) * sizeof(int));
@@ -199,7 +201,7 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     const long np = std::max(std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)),
                              fdk::scan_partials_needed(T)) + 1;
     a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));
-    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), a.nblkT) * sizeof(uint32_t));
+    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), 3 * a.nblkT) * sizeof(uint32_t));
     a.opCount = countOps ? (unsigned long long*)take(FD_NOPS * sizeof(unsigned long long)) : nullptr;
     if (!a.masks) a.masks = (uint32_t*)take(WH * a.L.res * a.L.wpr * sizeof(uint32_t));
     return off;
@@ -219,6 +221,7 @@ int frame_args(fd_ctx* c, FdArgs& a) {
     fill_args(a, c->models[0], c->pos, c->nrm, c->W, c->H, c->light, c->eps, c->cull, c->gridLog2, c->tileLog2, c->shadow,
               c->masks, c->pairCap);
     a.occVariant = c->occVariant;
+    a.triOrder = c->triOrder;
     return FD_OK;
 }
 
@@ -369,6 +372,13 @@ int fd_set_backface_cull(fd_ctx* c, int enable) {
 int fd_set_occlusion_variant(fd_ctx* c, int variant) {
     if (!c || (variant != FD_OCC_PER_SAMPLE && variant != FD_OCC_BIN_QUEUE)) return FD_ERR_ARG;
     c->occVariant = variant;
+    return FD_OK;
+}
+
+int fd_set_triangle_order(fd_ctx* c, int order) {
+    if (!c || (order != FD_TRI_ORDER_AUTO && order != FD_TRI_ORDER_LIST && order != FD_TRI_ORDER_AREA))
+        return FD_ERR_ARG;
+    c->triOrder = order;
     return FD_OK;
 }
 
@@ -721,6 +731,7 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
                 case FD_MAP_TRI: blocks = a.nblkT; break;
                 case FD_MAP_SCANT: blocks = fdk::scan_chunks(a.nT); break;
                 case FD_MAP_SCANO: blocks = fdk::scan_chunks(std::max(a.ntris, 1)); break;
+                case FD_MAP_NB: blocks = nb; break;
                 case FD_MAP_ZERO: blocks = a.segs > 1 ? fdk::zero_blocks(a) : 0; break;
                 case FD_MAP_FIN: blocks = a.segs > 1 ? a.chunkSlots : 0; break;
                 default: blocks = a.chunkSlots * a.segs; break;
@@ -779,6 +790,7 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
         const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
                   c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
+        b->args[i].triOrder = c->triOrder;
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
     }
@@ -1057,6 +1069,12 @@ extern "C" int fd_debug_read(fd_ctx* c, const char* name, void* dst, size_t maxb
     } else if (n == "pair_tri") {
         const size_t np = std::min<unsigned long long>(s.pairs, a.pairCap);
         get(a.sortedTri, np * sizeof(int));
+    } else if (n == "pair_tri_walk") {
+        const size_t np = std::min<unsigned long long>(s.pairs, a.pairCap);
+        get(s.status == FD_EXIT_OK && s.triOrder == FD_TRI_ORDER_AREA ? (const void*)a.pairKT : (const void*)a.sortedTri,
+            np * sizeof(int));
+    } else if (n == "tri_class") {
+        get(a.triClass, (size_t)a.ntris);
     } else {
         return FD_ERR_ARG;
     }

@@ -6,6 +6,8 @@
 
 #define FD_RXB 1024   // receivers per block in the key / scatter passes
 #define FD_TRB 1024   // triangles per block in the count / scatter passes
+#define FD_TRI_CLASSES 16      // area classes of the area walk order
+#define FD_ORDER_THREADS 1024  // threads per bin in the area-order pass
 
 // Everything one frame's kernels read, passed by value (kernarg segment).
 struct FdArgs {
@@ -46,7 +48,9 @@ struct FdArgs {
     uint2* triRange;      // ntris: (sx | sy << 16, ex | ey << 16)
     uint32_t* histT;      // nb * nblkT, scanned in place
     uint32_t* triOff;     // ntris: pair counts, scanned in place into triangle-major offsets
-    uint32_t* pairKT;     // pairCap: triangle-major pairs, (index in block << 16) | bin
+    uint32_t* pairKT;     // pairCap: triangle-major pairs, (index in block << 16) | bin; after the
+                          // scatter, the bins' lists in area-class order (when the frame uses it)
+    uint8_t* triClass;    // ntris: area class of each light-plane triangle's culling box (0 = largest)
     int* sortedTri;       // pairCap
     unsigned long long pairCap;
     int* rxStart;         // nb + 1
@@ -69,6 +73,9 @@ struct FdArgs {
     int binLds;
     // occlusion kernel: FD_OCC_PER_SAMPLE (the contract) or FD_OCC_BIN_QUEUE
     int occVariant;
+    // order of each bin's triangle list in the per-sample occlusion:
+    // FD_TRI_ORDER_AUTO, FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA
+    int triOrder;
 };
 
 // Waves per occlusion work item (per-sample kernel), for 32-bit and 64-bit
@@ -100,7 +107,7 @@ struct FdArgs {
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
 enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,
-             FD_MAP_OCC64, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO, FD_NMAPS };
+             FD_MAP_OCC64, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO, FD_MAP_NB, FD_NMAPS };
 
 struct FdBatchPlan {
     const FdArgs* inst;       // device: instance table (n)

@@ -53,6 +53,7 @@ __device__ __forceinline__ void init_body(const FdArgs& a, const int blk) {
     s->pairs = 0;
     s->rxTotal = 0;
     s->totalChunks = 0;
+    s->triOrder = FD_TRI_ORDER_LIST;
 }
 
 // Bin statistics reset (maxZ, minRange) for the bins of this frame.
@@ -628,9 +629,20 @@ __device__ __forceinline__ void walk_rect_wave(int sx, int sy, int ex, int ey, F
     }
 }
 
+// Area class of a triangle's light-plane culling box: 7 - floor(log2(area in
+// bin cells)) clamped to [0, FD_TRI_CLASSES - 1]; 0 holds the largest boxes
+// (64 cells and up), the last class boxes under 1/256 of a cell and the
+// degenerate ones (area zero, denormal or NaN).
+__device__ __forceinline__ int area_class(const float4 mn, const float4 mx, const FdState* s) {
+    const float cells = ((mx.x - mn.x) * s->factor[0]) * ((mx.y - mn.y) * s->factor[1]);
+    if (!(cells > 0.0f)) return FD_TRI_CLASSES - 1;
+    const int e = (int)(__float_as_uint(cells) >> 23) - 127;   // floor(log2), -127 for denormals
+    return min(max(7 - e, 0), FD_TRI_CLASSES - 1);
+}
+
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t thist[];  // nb counters, then the staged bin statistics
-    __shared__ uint32_t blkValid;
+    __shared__ uint32_t blkValid, blkPairs, blkOverlaps;
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
@@ -638,7 +650,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     float4* binQ = (float4*)(thist + nb);
     float* binW = (float*)(binQ + nb);
     for (int k = threadIdx.x; k < nb; k += blockDim.x) thist[k] = 0;
-    if (threadIdx.x == 0) blkValid = 0;
+    if (threadIdx.x == 0) blkValid = blkPairs = blkOverlaps = 0;
     if (staged) stage_bins(a, nb, binQ, binW);
     __syncthreads();
     const int t = blk * FD_TRB + threadIdx.x;
@@ -708,10 +720,23 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         }
         a.triAABB[2 * t] = mn;
         a.triAABB[2 * t + 1] = mx;
+        a.triClass[t] = (uint8_t)(valid ? area_class(mn, mx, s) : FD_TRI_CLASSES - 1);
     }
     {
         const int nv = wave_sum_i32(valid);
         if (lane_id() == 0 && nv) atomicAdd(&blkValid, (uint32_t)nv);
+        // list coherence: how often a valid triangle's culling box overlaps
+        // the next triangle's (the next lane's; the wave's last lane has none)
+        const float nx0 = __shfl_down(mn.x, 1, 64), ny0 = __shfl_down(mn.y, 1, 64);
+        const float nx1 = __shfl_down(mx.x, 1, 64), ny1 = __shfl_down(mx.y, 1, 64);
+        const int nvld = __shfl_down(valid, 1, 64);
+        const bool pair = valid && nvld && lane_id() < 63;
+        const bool over = pair && mn.x <= nx1 && nx0 <= mx.x && mn.y <= ny1 && ny0 <= mx.y;
+        const int np = __popcll(__ballot(pair)), no = __popcll(__ballot(over));
+        if (lane_id() == 0 && np) {
+            atomicAdd(&blkPairs, (uint32_t)np);
+            atomicAdd(&blkOverlaps, (uint32_t)no);
+        }
     }
     // count pass
     TriClamp c = tri_clamp(mn, mx, s);
@@ -762,7 +787,11 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
     }
     __syncthreads();
-    if (threadIdx.x == 0) a.part[blk] = blkValid;   // totalled in k_post_pairs
+    if (threadIdx.x == 0) {   // totalled in k_post_pairs
+        a.part[blk] = blkValid;
+        a.part[a.nblkT + blk] = blkPairs;
+        a.part[2 * a.nblkT + blk] = blkOverlaps;
+    }
     for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blk] = thist[k];
 }
 
@@ -1037,6 +1066,78 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
     }
 }
 
+// Area order of one bin's triangle list (frames whose walk order is
+// FD_TRI_ORDER_AREA): a stable counting sort of triStart..triEnd of the
+// reference-order list by area class, into the same range of pairKT (free
+// once the pairs are scattered).  One block of FD_ORDER_THREADS per bin:
+// pass 1 counts the classes, pass 2 ranks every entry by wave ballots plus an
+// exclusive prefix over the waves of each slice.
+__device__ __forceinline__ void tri_order_body(const FdArgs& a, const int bin) {
+    constexpr int NWV = FD_ORDER_THREADS / 64, C = FD_TRI_CLASSES;
+    __shared__ int waveCount[NWV][C];
+    __shared__ int classBase[C];
+    const FdState* s = a.st;
+    if (s->status != FD_EXIT_OK || s->triOrder != FD_TRI_ORDER_AREA || bin >= s->nb) return;
+    const int ts = a.triStart[bin], te = a.triEnd[bin];
+    if (te <= ts) return;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    int* out = (int*)a.pairKT;
+    auto cls_of = [&](int i, int& tri) -> int {
+        tri = a.sortedTri[i];
+        return ((unsigned)tri < (unsigned)a.ntris) ? (int)a.triClass[tri] : C - 1;
+    };
+    // pass 1: lane c of each wave counts class c
+    int mine = 0;
+    for (int i0 = ts; i0 < te; i0 += FD_ORDER_THREADS) {
+        const int i = i0 + threadIdx.x;
+        int tri = 0;
+        const int c = i < te ? cls_of(i, tri) : C;
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const int cnt = __popcll(__ballot(c == k));
+            if (lane == k) mine += cnt;
+        }
+    }
+    if (lane < C) waveCount[w][lane] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {   // exclusive prefix over the classes
+        int run = 0;
+        for (int k = 0; k < C; ++k) {
+            int tot = 0;
+            for (int j = 0; j < NWV; ++j) tot += waveCount[j][k];
+            classBase[k] = run;
+            run += tot;
+        }
+    }
+    __syncthreads();
+    // pass 2: stable scatter, one slice of FD_ORDER_THREADS entries at a time
+    for (int i0 = ts; i0 < te; i0 += FD_ORDER_THREADS) {
+        const int i = i0 + threadIdx.x;
+        int tri = 0;
+        const int c = i < te ? cls_of(i, tri) : C;
+        int rank = 0;
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const uint64_t m = __ballot(c == k);
+            if (c == k) rank = mbcnt(m);
+            if (lane == k) waveCount[w][k] = __popcll(m);
+        }
+        __syncthreads();
+        if (threadIdx.x < C) {   // per class: exclusive prefix over the waves, then advance the base
+            int run = classBase[threadIdx.x];
+            for (int j = 0; j < NWV; ++j) {
+                const int x = waveCount[j][threadIdx.x];
+                waveCount[j][threadIdx.x] = run;
+                run += x;
+            }
+            classBase[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (c < C) out[ts + waveCount[w][c] + rank] = tri;
+        __syncthreads();
+    }
+}
+
 // Exit bookkeeping after the scans (setupTriangleVertex returns 1 when no
 // triangle reached the light plane, Step0_triangleSetup.inl:224-225;
 // FDLaunch skips the rest when there are no pairs, FD/api.cpp:522).
@@ -1046,17 +1147,38 @@ __device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) 
     if (s->status != FD_EXIT_OK) return;
     // valid triangles: the pair-count blocks' totals (no count pass without
     // triangles)
-    __shared__ int wsum[FD_FIN_THREADS / 64];
-    int v = 0;
+    __shared__ int wsum[3][FD_FIN_THREADS / 64];
+    int v = 0, np = 0, no = 0;
     if (a.ntris > 0)
-        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) v += (int)a.part[i];
+        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) {
+            v += (int)a.part[i];
+            np += (int)a.part[a.nblkT + i];
+            no += (int)a.part[2 * a.nblkT + i];
+        }
     v = wave_sum_i32(v);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    np = wave_sum_i32(np);
+    no = wave_sum_i32(no);
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][threadIdx.x >> 6] = v;
+        wsum[1][threadIdx.x >> 6] = np;
+        wsum[2][threadIdx.x >> 6] = no;
+    }
     __syncthreads();
     if (threadIdx.x != 0) return;
     int nv = 0;
-    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) nv += wsum[j];
+    long pairsAdj = 0, overlaps = 0;
+    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) {
+        nv += wsum[0][j];
+        pairsAdj += wsum[1][j];
+        overlaps += wsum[2][j];
+    }
     s->validTris = nv;
+    // walk order of the bins' lists: the reference's (ascending id) when
+    // neighbouring triangles mostly overlap (a connected mesh: the list order
+    // already sweeps the surface), else largest culling boxes first (a
+    // triangle soup: big occluders fill receivers early)
+    s->triOrder = a.triOrder != FD_TRI_ORDER_AUTO ? a.triOrder
+                  : (2 * overlaps >= pairsAdj ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_AREA);
     // valid receivers: binned (keys < nb, the scanned histogram at key nb)
     // plus keys out of range
     s->validSamples = (int)a.histR[(size_t)s->nb * a.nblkR] + s->keyOutOfRange;
@@ -1693,7 +1815,8 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     typedef const int* cint_p;   // host pass: the kernel body is never run
     typedef const FdTri* ctri_p;
 #endif
-    const cint_p triList = (cint_p)(a.sortedTri);
+    // the bin's list in the frame's walk order (the masks do not depend on it)
+    const cint_p triList = (cint_p)(s->triOrder == FD_TRI_ORDER_AREA ? (const int*)a.pairKT : a.sortedTri);
     const ctri_p tris = (ctri_p)(a.tri);
     bool live = act;   // receiver in this wave and not yet fully shadowed
     for (int ti = ts + wv; ti < te; ti += NW) {
@@ -1982,6 +2105,7 @@ __global__ __launch_bounds__(FD_TRB) void k_tri_write(FdArgs a) { tri_write_body
 __global__ __launch_bounds__(FD_TRB) void k_tri_scatter_wide(FdArgs a) { tri_scatter_wide_body(a, blockIdx.x); }
 __global__ __launch_bounds__(1024) void k_tables(FdArgs a) { tables_body(a, blockIdx.x); }
 __global__ __launch_bounds__(FD_FIN_THREADS) void k_post_pairs(FdArgs a) { post_pairs_body(a, blockIdx.x); }
+__global__ __launch_bounds__(FD_ORDER_THREADS) void k_tri_order(FdArgs a) { tri_order_body(a, blockIdx.x); }
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
 // b % 8).  Work items of one bin share that bin's triangle list, so the
 // occlusion kernels renumber blocks so that each XCD takes runs of 16
@@ -2060,6 +2184,7 @@ FD_BATCHED(kb_tri_write, FD_TRB, tri_write_body)
 FD_BATCHED(kb_tri_scatter_wide, FD_TRB, tri_scatter_wide_body)
 FD_BATCHED(kb_tables, 1024, tables_body)
 FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
+FD_BATCHED(kb_tri_order, FD_ORDER_THREADS, tri_order_body)
 #undef FD_BATCHED
 
 template <typename T>
@@ -2264,6 +2389,7 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
         kb_tri_scatter_wide<<<L(FD_MAP_TRI), FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(p.inst, p.map[FD_MAP_TRI], n);
     }
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
+    if (p.anyTris) kb_tri_order<<<L(FD_MAP_NB), FD_ORDER_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_NB], n);
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
     for (int c = 0; c < 2; ++c) {
         if (p.nocc[c] == 0) continue;
@@ -2346,6 +2472,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     }
     mark(3);
     k_tables<<<1, 1024, 0, st>>>(a);
+    if (a.ntris > 0) k_tri_order<<<nb, FD_ORDER_THREADS, 0, st>>>(a);
     mark(4);
     // occlusion (or the early-exit fill); split lists zero the masks first
     // and count them after the merge

@@ -29,6 +29,7 @@ EXIT_OK, EXIT_SAMPLE_MAXZ, EXIT_TRI_MAXZ, EXIT_NO_VALID_SAMPLE = 0, 1, 2, 3
 EXIT_EMPTY_AABB, EXIT_EMPTY_LP, EXIT_NO_VALID_TRI, EXIT_NO_PAIRS, EXIT_CAPACITY = 4, 5, 6, 7, 9
 
 OCC_PER_SAMPLE, OCC_BIN_QUEUE = 0, 1   # fd_set_occlusion_variant
+TRI_ORDER_AUTO, TRI_ORDER_LIST, TRI_ORDER_AREA = 0, 1, 2   # fd_set_triangle_order
 
 
 class AreaLight(ctypes.Structure):
@@ -129,6 +130,7 @@ def lib():
             "fd_set_bin_num": ([vp, sz], ip),
             "fd_set_tile_width": ([vp, ip], ip),
             "fd_set_occlusion_variant": ([vp, ip], ip),
+            "fd_set_triangle_order": ([vp, ip], ip),
             "fd_autotune": ([vp, ctypes.c_void_p, ctypes.c_void_p, ip, ip, ctypes.POINTER(ip)], ip),
             "fd_set_view": ([vp, ip, ip], ip),
             "fd_set_scenes_esp": ([vp, ctypes.c_float], ip),
@@ -329,6 +331,11 @@ class Context:
         or OCC_BIN_QUEUE (its unused shadowCal_CTA_kernel; other arithmetic,
         mask rows along y_delt)."""
         _check(lib().fd_set_occlusion_variant(self.h, int(variant)), f"fd_set_occlusion_variant({variant})")
+
+    def set_triangle_order(self, order):
+        """TRI_ORDER_AUTO / TRI_ORDER_LIST / TRI_ORDER_AREA: the order in which
+        the occlusion walks each bin's triangle list (changes no result)."""
+        _check(lib().fd_set_triangle_order(self.h, int(order)), f"fd_set_triangle_order({order})")
 
     def autotune(self, inst_item, candidates, reps=3):
         """Autotuner (fd_autotune): inst_item as a Batch item (its outputs are
@@ -568,7 +575,7 @@ STATE_DTYPE = np.dtype([
     ("vxZmax", "<u4"), ("vxMin", "<u4", 3), ("vxMaxA", "<u4", 2), ("vxMaxZ", "<u4"),
     ("validRect", "<i4"), ("status", "<i4"), ("validSamples", "<i4"), ("validTris", "<i4"),
     ("keyOutOfRange", "<i4"), ("capacityHit", "<i4"), ("pairs", "<u8"), ("rxTotal", "<u8"),
-    ("totalChunks", "<i4"), ("pad0", "<i4"), ("sampleMaxZ", "<f4"), ("triMaxZ", "<f4"), ("f", "<f4"),
+    ("totalChunks", "<i4"), ("triOrder", "<i4"), ("sampleMaxZ", "<f4"), ("triMaxZ", "<f4"), ("f", "<f4"),
     ("S", "<f4", 2), ("sAABB", "<f4", 8), ("mAABB", "<f4", 8), ("B", "<f4", 3), ("E", "<f4", 3),
     ("factor", "<f4", 2), ("G", "<i4"), ("log2G", "<i4"), ("nb", "<i4"), ("pad1", "<i4")], align=True)
 

@@ -108,7 +108,7 @@ def gpu_plain(pkg, frame, steps, warmup, quick=False):
         src.close()
         dst.close()
         return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=[ms_per_step], stage=[0.0] * 5,
-                    ops=[0] * 8, stats=st.as_dict(), shadow=None, masks=None)
+                    ops=[0] * 8, ops_ref=[0] * 8, walk_order=0, stats=st.as_dict(), shadow=None, masks=None)
     # latency: one frame at a time, host wall clock around fd_launch
     lat = []
     for _ in range(max(steps, 10)):
@@ -122,17 +122,24 @@ def gpu_plain(pkg, frame, steps, warmup, quick=False):
         ctx.launch_async(stream)
         stage.append(ctx.get_profile()[0])
     stage = np.median(np.asarray(stage), axis=0)
-    # occlusion work counters (a separate frame: counting adds instructions)
+    walk_order = int(fd.read_state(ctx)["triOrder"])
+    # occlusion work counters (separate frames: counting adds instructions),
+    # in the order the frame walks the bins' lists and in the reference's
+    # (ascending triangle id)
     ctx.set_profiling(False, True)
     ctx.launch_async(stream)
     _, ops = ctx.get_profile(ops=True)
+    ctx.set_triangle_order(fd.TRI_ORDER_LIST)
+    ctx.launch_async(stream)
+    _, ops_ref = ctx.get_profile(ops=True)
+    ctx.set_triangle_order(fd.TRI_ORDER_AUTO)
     ctx.set_profiling(False, False)
     # the outputs checked below come from the timed kernel, not the counting one
     ctx.launch_async(stream)
     strm.sync()
     shadow, masks = run.outputs()
     return dict(run=run, bits=bits, ms_per_step=ms_per_step, lat=lat, stage=stage.tolist(), ops=ops,
-                stats=st.as_dict(), shadow=shadow, masks=masks)
+                ops_ref=ops_ref, walk_order=walk_order, stats=st.as_dict(), shadow=shadow, masks=masks)
 
 
 def make_frame(wl, workload, light_res):
@@ -142,11 +149,13 @@ def make_frame(wl, workload, light_res):
 
 def occlusion_roofline(g, occ_ms, workload, light_res):
     """Roofline object of the occlusion kernel: algorithmic operations per
-    launch from the kernel's counters (the reference's rows and, separately,
-    the rows this kernel visits) over the kernel's event-timed duration,
-    against the FP32 vector peak."""
+    launch from the kernel's counters (the reference's work: its list order
+    and the rows it visits; and, separately, the work this kernel does in
+    its walk order with its row window) over the kernel's event-timed
+    duration, against the FP32 vector peak."""
     cand, npass, rows, items, wave_it, wave_rows, ref_rows, planes = g["ops"]
-    ops_ref = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * ref_rows
+    rcand, rpass, _, _, rwave_it, _, rref_rows, rplanes = g["ops_ref"]
+    ops_ref = OPS_PER_CAND * rcand + OPS_PER_PLANES * rplanes + OPS_PER_ROW * rref_rows
     ops_run = OPS_PER_CAND * cand + OPS_PER_PLANES * planes + OPS_PER_ROW * rows
     achieved = ops_ref / (occ_ms * 1e-3) / 1e12
     achieved_run = ops_run / (occ_ms * 1e-3) / 1e12
@@ -157,18 +166,21 @@ def occlusion_roofline(g, occ_ms, workload, light_res):
                 achieved_kernel_rows=round(achieved_run, 3),
                 frac_kernel_rows=round(achieved_run / FP32_VECTOR_PEAK_TFLOPS, 4),
                 work_model=("achieved: 5 ops per candidate test + 111 per pair whose planes are set up + 15 per "
-                            "light row the reference visits (rows this kernel skips by its row window count); "
-                            "achieved_kernel_rows: the same with the rows this kernel visits"),
+                            "light row the reference visits, all counted in the reference's list order (work this "
+                            "kernel skips by its walk order and row window counts); achieved_kernel_rows: the "
+                            "same with the tests, planes and rows this kernel does in its walk order"),
                 traffic_source=prof["source"] if prof else None,
                 valu_issue_profiled=prof.get("valu_issue") if prof else None,
                 occupancy_profiled=prof.get("occupancy") if prof else None,
                 kernel="k_occlude", kernel_ms=round(occ_ms, 4),
+                walk_order={1: "list", 2: "area classes"}.get(g["walk_order"], "n/a"),
                 work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
                           work_items=int(items), wave_tri_iterations=int(wave_it),
                           wave_row_iterations=int(wave_rows),
-                          lane_util_rows=round(rows / max(1, 64 * wave_rows), 4),
-                          reference_row_visits=int(ref_rows), planes=int(planes),
-                          ops_per_event=[OPS_PER_CAND, OPS_PER_PLANES, OPS_PER_ROW]))
+                          lane_util_rows=round(rows / max(1, 64 * wave_rows), 4), planes=int(planes),
+                          ops_per_event=[OPS_PER_CAND, OPS_PER_PLANES, OPS_PER_ROW]),
+                reference_work=dict(candidate_tests=int(rcand), cull_passes=int(rpass), planes=int(rplanes),
+                                    row_visits=int(rref_rows), wave_tri_iterations=int(rwave_it)))
 
 
 def plain_leg(pkg, frame, workload, steps, warmup, quick):

@@ -137,6 +137,25 @@ int fd_set_tile_width(fd_ctx* ctx, int tile_width);
 #define FD_OCC_BIN_QUEUE 1
 int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
 
+/* No counterpart (a schedule knob of this implementation): the order in
+ * which the per-sample occlusion walks each bin's triangle list.  Masks and
+ * shadow values do not depend on it (each receiver's mask is the OR of its
+ * triangles' masks; rows already full are skipped, which changes nothing);
+ * the order only decides how soon receivers fill up and stop.
+ *   FD_TRI_ORDER_LIST: the reference's order (ascending triangle id, the
+ *     stable sort of FDc/Step4_RasterizeTriangleToBin.inl:223);
+ *   FD_TRI_ORDER_AREA: by area of the triangle's light-plane culling box in
+ *     16 powers of two, largest first, ascending id within a class;
+ *   FD_TRI_ORDER_AUTO (default): decided per frame on the device -- the
+ *     list order when consecutive triangles mostly overlap on the light
+ *     plane (a connected mesh), else the area order (a triangle soup).
+ * The batched and serving forms use the context's / the default setting;
+ * fd_debug_read("pair_tri") always returns the reference's order. */
+#define FD_TRI_ORDER_AUTO 0
+#define FD_TRI_ORDER_LIST 1
+#define FD_TRI_ORDER_AREA 2
+int fd_set_triangle_order(fd_ctx* ctx, int order);
+
 /* FD::FDsetView, FD/api.cpp:205-237. */
 int fd_set_view(fd_ctx* ctx, int width, int height);
 
@@ -386,7 +405,10 @@ int fd_mask_words(int light_res);
  * "tri_count", "tri_range", "pair_tri", "bin_tri_start", "bin_tri_end",
  * "light" (light block floats), "state" (raw state).  "sorted_id" lists each
  * bin's receivers by ascending id (the reference's stable order; the kernels
- * themselves walk a bin in 8x8-pixel tile order).  Synchronises the device.
+ * themselves walk a bin in 8x8-pixel tile order).  No reference counterpart:
+ * "tri_class" (uint8 area class per triangle) and "pair_tri_walk" (the bins'
+ * lists in the order the occlusion walked them, see fd_set_triangle_order).
+ * Synchronises the device.
  * *bytes receives the size; copies min(size, max_bytes). */
 int fd_debug_read(fd_ctx* ctx, const char* name, void* host_dst, size_t max_bytes, size_t* bytes);
 

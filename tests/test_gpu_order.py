@@ -1,0 +1,164 @@
+"""Walk order of the bins' triangle lists in the occlusion (MI355X only).
+
+fd_set_triangle_order picks the order in which the occlusion kernel walks
+each bin's list: the reference's (ascending triangle id), largest culling
+boxes first (16 power-of-two area classes), or per frame on the device.  A
+receiver's mask is the OR of its triangles' masks, so the order must not
+change a bit of any output; it only changes how soon receivers fill up.
+
+Checked here: the automatic choice (area order for a triangle soup, list
+order for connected meshes), oracle bits under every order, the area-ordered
+lists as a stable class sort of the reference's lists with classes that obey
+their definition, and the batched form over a soup.
+"""
+import numpy as np
+import pytest
+import xxhash
+
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+LIST, AREA, AUTO = 1, 2, 0
+
+
+def _soup(wl, light_res=64):
+    # boxes-1080p's generator at 480x270 receivers and 150k triangles
+    # (8 Gaussian clusters of independent triangles)
+    return wl.boxes_1080p(w=480, h=270, ntri=150_000, light_res=light_res)
+
+
+def _mesh(wl):
+    return wl.heightfield_1k(res=256, nsph=100)
+
+
+def _digest(sh, mk):
+    h = xxhash.xxh3_128()
+    h.update(np.ascontiguousarray(sh).tobytes())
+    h.update(np.ascontiguousarray(mk).tobytes())
+    return h.hexdigest()
+
+
+def _run(fd, frame, order):
+    fr = fd.FrameRunner(frame)
+    fr.ctx.set_triangle_order(order)
+    rc, st = fr.run()
+    assert rc == 0 and st.exit_code == 0
+    return fr
+
+
+@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("mesh", LIST)])
+def test_auto_choice_and_oracle(fd, wl, oracle, which, expect):
+    frame = {"soup": lambda: _soup(wl), "soup32": lambda: _soup(wl, 32), "mesh": lambda: _mesh(wl)}[which]()
+    print(f"[order] auto {which}", flush=True)
+    fr = _run(fd, frame, AUTO)
+    assert int(fd.read_state(fr.ctx)["triOrder"]) == expect
+    rep, fr, o = compare(fd, oracle, frame, fr=fr)
+    bad = {k: v for k, v in rep.items() if v}
+    assert not bad, bad
+    fr.close()
+
+
+@pytest.mark.parametrize("which", ["soup", "mesh"])
+def test_orders_give_identical_bits(fd, wl, which):
+    frame = _soup(wl) if which == "soup" else _mesh(wl)
+    print(f"[order] forced orders {which}", flush=True)
+    digests, pair_lists = {}, {}
+    for order in (LIST, AREA, AUTO):
+        fr = _run(fd, frame, order)
+        chosen = int(fd.read_state(fr.ctx)["triOrder"])
+        assert chosen == (order if order else chosen) and chosen in (LIST, AREA)
+        sh, mk = fr.outputs()
+        digests[order] = _digest(sh, mk)
+        pair_lists[order] = fr.ctx.debug_read("pair_tri", np.int32)   # always the reference's order
+        walk = fr.ctx.debug_read("pair_tri_walk", np.int32)
+        if chosen == LIST:
+            assert np.array_equal(walk, pair_lists[order])
+        fr.close()
+    assert len(set(digests.values())) == 1, digests
+    assert np.array_equal(pair_lists[LIST], pair_lists[AREA]) and np.array_equal(pair_lists[LIST], pair_lists[AUTO])
+
+
+def test_area_lists_are_stable_class_sort(fd, wl):
+    frame = _soup(wl)
+    print("[order] area lists", flush=True)
+    fr = _run(fd, frame, AREA)
+    ctx = fr.ctx
+    s = fd.read_state(ctx)
+    assert int(s["triOrder"]) == AREA
+    cls = ctx.debug_read("tri_class", np.uint8)
+    aabb = ctx.debug_read("tri_aabb", np.float32).reshape(-1, 2, 4)
+    pairs = ctx.debug_read("pair_tri", np.int32)
+    walk = ctx.debug_read("pair_tri_walk", np.int32)
+    ts = ctx.debug_read("bin_tri_start", np.int32)
+    te = ctx.debug_read("bin_tri_end", np.int32)
+    fr.close()
+    assert len(pairs) == int(s["pairs"]) and len(walk) == len(pairs)
+    assert cls.max() < 16
+    # classes against their definition: class k in 1..14 holds boxes of
+    # 2^(7-k) <= area < 2^(8-k) bin cells, class 0 every area >= 64 cells,
+    # class 15 every area < 2^-7 cells (area in float64; boxes within 1e-5 of
+    # a class boundary may fall either side under float32 rounding)
+    used = np.unique(pairs)
+    mn, mx = aabb[used, 0].astype(np.float64), aabb[used, 1].astype(np.float64)
+    cells = (mx[:, 0] - mn[:, 0]) * float(s["factor"][0]) * (mx[:, 1] - mn[:, 1]) * float(s["factor"][1])
+    c = cls[used].astype(np.int64)
+    lo = np.where(c == 15, 0.0, 2.0 ** (7 - c))
+    hi = np.where(c == 0, np.inf, 2.0 ** (8 - c))
+    ok = (cells >= lo * (1 - 1e-5)) & (cells < hi * (1 + 1e-5))
+    assert ok.all(), (int((~ok).sum()), cells[~ok][:5], c[~ok][:5])
+    assert len(np.unique(c)) >= 4   # the soup spans several classes
+    # each bin's walked list: the reference's list stably sorted by class
+    nonempty = 0
+    for b in range(len(ts)):
+        if te[b] <= ts[b]:
+            continue
+        nonempty += 1
+        ref = pairs[ts[b]:te[b]]
+        exp = ref[np.argsort(cls[ref], kind="stable")]
+        assert np.array_equal(walk[ts[b]:te[b]], exp), b
+    assert nonempty > 0
+
+
+def test_batch_over_soup(fd, wl):
+    # the batched form with the default (automatic) order over a soup scene,
+    # instance by instance the same bits as the plain path in list order
+    scene = wl.tris_clusters(60_000, 11)
+    ctx = fd.Context()
+    ctx.set_backface_cull(1)
+    ctx.set_scenes_esp(1e-3)
+    verts, idx = fd.DeviceBuffer.from_array(scene.verts), fd.DeviceBuffer.from_array(scene.idx)
+    ctx.add_model(verts, scene.nverts, 16, idx, scene.ntris)
+    items = []
+    for (w, h, R, c) in [(160, 90, 16, (0.0, 0.0, 6.0)), (320, 180, 33, (1.0, -0.5, 7.0)), (256, 256, 64, (-1.0, 1.0, 6.5))]:
+        rx = wl.receivers_boxes(w, h, 20, 3)
+        items.append(dict(w=w, h=h, bins=0, rx=rx, light=wl.LightDesc(1.5, 0.5, c, res_w=R, res_h=R),
+                          pos=fd.DeviceBuffer.from_array(rx.pos), nrm=fd.DeviceBuffer.from_array(rx.nrm),
+                          shadow=fd.DeviceBuffer(w * h * 4), masks=fd.DeviceBuffer(w * h * fd.mask_words(R) * 4)))
+    alone, areas = [], 0
+    for it in items:
+        fr = wl.Frame("alone", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1)
+        run = fd.FrameRunner(fr)
+        run.ctx.set_triangle_order(LIST)
+        rc, st = run.run()
+        assert rc == 0 and st.exit_code == 0
+        alone.append(_digest(*run.outputs()))
+        run.ctx.set_triangle_order(AUTO)
+        rc, st = run.run()
+        areas += int(fd.read_state(run.ctx)["triOrder"]) == AREA
+        run.close()
+    assert areas == len(items)   # the automatic choice takes the area order on every instance
+    print("[order] batch over soup", flush=True)
+    stream = fd.Stream()
+    b = fd.Batch(ctx, items)
+    b.launch(stream.handle)
+    stream.sync()
+    for k, it in enumerate(items):
+        rc, st = b.stats(k)
+        assert rc == 0 and st.exit_code == 0, k
+        sh = it["shadow"].download(np.float32)
+        mk = it["masks"].download(np.uint32)
+        assert _digest(sh, mk) == alone[k], k
+    b.close()
+    stream.close()
+    ctx.close()

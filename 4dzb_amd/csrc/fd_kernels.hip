@@ -1815,21 +1815,38 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     typedef const int* cint_p;   // host pass: the kernel body is never run
     typedef const FdTri* ctri_p;
 #endif
-    // the bin's list in the frame's walk order (the masks do not depend on it)
-    const cint_p triList = (cint_p)(s->triOrder == FD_TRI_ORDER_AREA ? (const int*)a.pairKT : a.sortedTri);
+    // the bin's list in the frame's walk order (the masks do not depend on it);
+    // the order is read once into a scalar register, so the list address
+    // stays scalar
+    const int walkOrder = uniform(s->triOrder);
+    const cint_p triList = (cint_p)(walkOrder == FD_TRI_ORDER_AREA ? (const int*)a.pairKT : a.sortedTri);
     const ctri_p tris = (ctri_p)(a.tri);
     bool live = act;   // receiver in this wave and not yet fully shadowed
-    for (int ti = ts + wv; ti < te; ti += NW) {
-        const uint64_t liveMask = __ballot(live);
-        if (liveMask == 0) break;
-        if (COUNT) { nCand += __popcll(liveMask); ++nWaveIt; }
-        const int tid = triList[ti];
-        if ((unsigned)tid >= (unsigned)a.ntris) continue;   // defensive: pair entries are triangle ids
+    // Prefilter (timed kernel): the wave reads its next 64 list entries and
+    // their culling boxes with vector loads and keeps the triangles that pass
+    // the containment + depth test against bounds of all its receivers (the
+    // largest lower corner, the smallest upper corner, the largest depth).  A
+    // triangle that fails them fails the test of every lane, which then skips
+    // it: dropping it early changes nothing.  The counting kernel walks every
+    // entry, as the reference does.
+    const int* listV = walkOrder == FD_TRI_ORDER_AREA ? (const int*)a.pairKT : a.sortedTri;
+    float gx0 = 0.0f, gy0 = 0.0f, gx1 = 0.0f, gy1 = 0.0f, gz = 0.0f;
+    if (!COUNT) {
+        auto su = [](float v) { return __int_as_float(uniform(__float_as_int(v))); };   // into a scalar register
+        gx0 = su(wave_max_f32(act ? q.rminx : -INFINITY));
+        gy0 = su(wave_max_f32(act ? q.rminy : -INFINITY));
+        gx1 = su(wave_min_f32(act ? q.rmaxx : INFINITY));
+        gy1 = su(wave_min_f32(act ? q.rmaxy : INFINITY));
+        gz = su(wave_max_f32(act ? q.z : -INFINITY));
+    }
+    // one pair (triangle tid, this wave's receivers): cull, planes, rows
+    auto step = [&](const int tid) {
+        if ((unsigned)tid >= (unsigned)a.ntris) return;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
         // containment + depth cull (:120-127)
         const bool pass = live && t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx &&
                           q.rmaxy <= t.bmaxy && t.minz <= q.z;
-        if (!pass) continue;
+        if (!pass) return;
         if (COUNT) ++nPass;
         // planes (Q,v0,v1), (Q,v1,v2), (Q,v2,v0) (:150-172) start from
         // a_k = v_{k+1} - Q; a_2 = v0 - Q also gives
@@ -1850,7 +1867,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         // variant tests below, only for pairs that can still set a bit (the
         // outcome is the same: a skipped pair sets nothing).
         const bool onTri = fabsf(ndir) < eps;
-        if (COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+        if (COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) return;
         // The reference negates t0, t1, L0_Q when dir < 0.  The sign is put
         // into L0 - Q and the light axes before the products (a sign flip
         // commutes with every rounding): A, B, C come out negated, identical
@@ -1976,10 +1993,10 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             // window; a wave none of whose lanes keeps a row skips the
             // triangle at either step (the cheaper test first)
             T valid = (~full) & FULL & three_edges_fast<T>(DN, nR, rA, dA, Rs, R);
-            if (!__any(valid != (T)0)) continue;
+            if (!__any(valid != (T)0)) return;
             valid &= row_window<T>(e, Rf, mAB);
-            if (!__any(valid != (T)0)) continue;   // no lane can gain a bit from this triangle
-            if (!COUNT && onTri && valid != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+            if (!__any(valid != (T)0)) return;   // no lane can gain a bit from this triangle
+            if (!COUNT && onTri && valid != (T)0 && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) return;
             float nC[3], rB[3], dB[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -1991,7 +2008,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             wave_rows(valid);
             rows(valid, [&](const float (&D)[3]) { return three_edges_fast<T>(D, nC, rB, dB, Rs, R); });
         } else {
-            if (!COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) continue;
+            if (!COUNT && onTri && point_in_tri(t.v0, t.v1, t.v2, q.q[0], q.q[1], q.q[2])) return;
             float DN[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -2006,6 +2023,37 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
             rows(valid, [&](const float (&D)[3]) { return (T)(FULL & three_edges<T, false>(e, D, Rf, R)); });
         }
         if (full == FULL) live = false;
+    };
+    if (COUNT) {
+        for (int ti = ts + wv; ti < te; ti += NW) {
+            const uint64_t liveMask = __ballot(live);
+            if (liveMask == 0) break;
+            nCand += __popcll(liveMask);
+            ++nWaveIt;
+            step(triList[ti]);
+        }
+    } else {
+        // blocks of 64 of this wave's entries; the candidate loop reads the
+        // block's ids from the lanes that loaded them (written once per
+        // block, so any subset of lanes that runs an iteration reads the
+        // same id)
+        for (int blk0 = ts + wv; blk0 < te; blk0 += 64 * NW) {
+            if (__ballot(live) == 0) break;
+            const int tj = blk0 + lane * NW;
+            int tidv = -1;
+            bool keep = false;
+            if (tj < te) {
+                tidv = listV[tj];
+                if ((unsigned)tidv < (unsigned)a.ntris) {
+                    const float4 bmn = a.triAABB[2 * tidv], bmx = a.triAABB[2 * tidv + 1];
+                    keep = bmn.x <= gx0 && bmn.y <= gy0 && gx1 <= bmx.x && gy1 <= bmx.y && bmn.z <= gz;
+                }
+            }
+            for (uint64_t cmask = __ballot(keep); cmask; cmask &= cmask - 1) {
+                if (__ballot(live) == 0) break;
+                step(__builtin_amdgcn_readlane(tidv, uniform(__ffsll((unsigned long long)cmask) - 1)));
+            }
+        }
     }
     if (COUNT) {
         // per-lane counts: nCand is wave-wide already

@@ -85,7 +85,7 @@ struct FdArgs {
 #define FD_OCC_WAVES 4
 #endif
 #ifndef FD_OCC_WAVES64
-#define FD_OCC_WAVES64 8
+#define FD_OCC_WAVES64 4
 #endif
 
 

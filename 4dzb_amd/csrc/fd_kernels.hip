@@ -1843,6 +1843,9 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     auto step = [&](const int tid) {
         if ((unsigned)tid >= (unsigned)a.ntris) return;   // defensive: pair entries are triangle ids
         const FdTri t = tris[tid];
+        // the whole record in one round trip (the prefiltered pairs mostly
+        // pass the cull below, which needs only part of it)
+        if (!COUNT) asm volatile("" ::"s"(t.v0[0]), "s"(t.n[0]), "s"(t.e20[0]));
         // containment + depth cull (:120-127)
         const bool pass = live && t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx &&
                           q.rmaxy <= t.bmaxy && t.minz <= q.z;
@@ -2036,19 +2039,24 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         // blocks of 64 of this wave's entries; the candidate loop reads the
         // block's ids from the lanes that loaded them (written once per
         // block, so any subset of lanes that runs an iteration reads the
-        // same id)
+        // same id).  The next block's ids load while this block's boxes are
+        // tested and its pairs run; the box fields load unconditionally, in
+        // one round trip.
+        int tidNext = (ts + wv + lane * NW < te) ? listV[ts + wv + lane * NW] : -1;
         for (int blk0 = ts + wv; blk0 < te; blk0 += 64 * NW) {
             if (__ballot(live) == 0) break;
             const int tj = blk0 + lane * NW;
-            int tidv = -1;
+            const int tidv = tidNext;
             bool keep = false;
-            if (tj < te) {
-                tidv = listV[tj];
-                if ((unsigned)tidv < (unsigned)a.ntris) {
-                    const float4 bmn = a.triAABB[2 * tidv], bmx = a.triAABB[2 * tidv + 1];
-                    keep = bmn.x <= gx0 && bmn.y <= gy0 && gx1 <= bmx.x && gy1 <= bmx.y && bmn.z <= gz;
-                }
+            float4 bmn = make_float4(0.f, 0.f, 0.f, 0.f), bmx = bmn;
+            const bool okv = (unsigned)tidv < (unsigned)a.ntris;
+            if (okv) {
+                bmn = a.triAABB[2 * tidv];
+                bmx = a.triAABB[2 * tidv + 1];
             }
+            const int tn = tj + 64 * NW;
+            tidNext = tn < te ? listV[tn] : -1;
+            if (okv) keep = bmn.x <= gx0 && bmn.y <= gy0 && gx1 <= bmx.x && gy1 <= bmx.y && bmn.z <= gz;
             for (uint64_t cmask = __ballot(keep); cmask; cmask &= cmask - 1) {
                 if (__ballot(live) == 0) break;
                 step(__builtin_amdgcn_readlane(tidv, uniform(__ffsll((unsigned long long)cmask) - 1)));

@@ -92,6 +92,7 @@ struct fd_ctx {
     int tileLog2 = 3;     // 8x8 receiver tiles
     int occVariant = FD_OCC_PER_SAMPLE;
     int triOrder = FD_TRI_ORDER_AUTO;
+    int occWaves = FD_OCC_WAVES_DEFAULT;
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
     std::vector<Model> models;
@@ -222,6 +223,7 @@ int frame_args(fd_ctx* c, FdArgs& a) {
               c->masks, c->pairCap);
     a.occVariant = c->occVariant;
     a.triOrder = c->triOrder;
+    a.occWaves = c->occWaves;
     return FD_OK;
 }
 
@@ -372,6 +374,12 @@ int fd_set_backface_cull(fd_ctx* c, int enable) {
 int fd_set_occlusion_variant(fd_ctx* c, int variant) {
     if (!c || (variant != FD_OCC_PER_SAMPLE && variant != FD_OCC_BIN_QUEUE)) return FD_ERR_ARG;
     c->occVariant = variant;
+    return FD_OK;
+}
+
+int fd_set_occlusion_waves(fd_ctx* c, int waves) {
+    if (!c || (waves != 0 && waves != 2 && waves != 4)) return FD_ERR_ARG;
+    c->occWaves = waves ? waves : FD_OCC_WAVES_DEFAULT;
     return FD_OK;
 }
 
@@ -671,12 +679,12 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
     P.n = n;
     P.maxNb = 0;
     P.anyTris = anyTris;
-    std::vector<int> occIdx[2];
+    std::vector<int> occIdx[FD_OCC_CLASSES];
     for (int i = 0; i < n; ++i) {
         P.maxNb = std::max(P.maxNb, 1 << (2 * args[i].gridLog2));
-        occIdx[args[i].L.wpr - 1].push_back(i);
+        occIdx[fd_occ_class(args[i].L.wpr, args[i].occWaves)].push_back(i);
     }
-    for (int cl = 0; cl < 2; ++cl) {  // largest work first: shorter tail
+    for (int cl = 0; cl < FD_OCC_CLASSES; ++cl) {  // largest work first: shorter tail
         auto cost = [&](int i) { return (double)args[i].WH * args[i].L.res * args[i].L.res; };
         std::stable_sort(occIdx[cl].begin(), occIdx[cl].end(), [&](int x, int y) { return cost(x) > cost(y); });
         P.nocc[cl] = (int)occIdx[cl].size();
@@ -696,21 +704,21 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
         hArgs[i].binLds = P.maxNb <= FD_LDS_BINS;   // the batched launch sizes LDS for the largest grid
     }
     int k = n;
-    for (int cl = 0; cl < 2; ++cl)
+    for (int cl = 0; cl < FD_OCC_CLASSES; ++cl)
         for (int i : occIdx[cl]) hArgs[k++] = hArgs[i];
     int* hMaps = (int*)(image.data() + argBytes);
     const FdArgs* dArgs = (const FdArgs*)devBase;
     const int* dMaps = (const int*)(devBase + argBytes);
     P.inst = dArgs;
-    P.occ[0] = dArgs + n;
-    P.occ[1] = dArgs + n + P.nocc[0];
+    // the occlusion table holds the classes one after the other
+    for (int cl = 0, base = n; cl < FD_OCC_CLASSES; base += P.nocc[cl++]) P.occ[cl] = dArgs + base;
     for (int m = 0; m < FD_NMAPS; ++m) {
-        const bool occ = (m == FD_MAP_OCC32 || m == FD_MAP_OCC64 || m == FD_MAP_ZERO || m == FD_MAP_FIN);
-        const std::vector<int>* order = occ ? &occIdx[m == FD_MAP_OCC64 ? 1 : 0] : nullptr;
+        const bool occCls = m >= FD_MAP_OCC0 && m < FD_MAP_OCC0 + FD_OCC_CLASSES;
+        const bool occ = occCls || m == FD_MAP_ZERO || m == FD_MAP_FIN;
+        const std::vector<int>* order = occCls ? &occIdx[m - FD_MAP_OCC0] : nullptr;
         std::vector<int> all;   // zero / finish maps: every instance, in occlusion-table order
         if (m == FD_MAP_ZERO || m == FD_MAP_FIN) {
-            all = occIdx[0];
-            all.insert(all.end(), occIdx[1].begin(), occIdx[1].end());
+            for (int cl = 0; cl < FD_OCC_CLASSES; ++cl) all.insert(all.end(), occIdx[cl].begin(), occIdx[cl].end());
             order = &all;
         }
         const int cnt = occ ? (int)order->size() : n;
@@ -787,10 +795,12 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
         }
         FdLight L;
         light_setup(q.light, L);
+        if (q.occ_waves != 0 && q.occ_waves != 2 && q.occ_waves != 4) { delete b; return FD_ERR_ARG; }
         const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
                   c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
         b->args[i].triOrder = c->triOrder;
+        b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves;
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
     }
@@ -879,10 +889,11 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
     // one layout: returns a status, the median device time and the outputs
-    auto run = [&](int bins, int tw, uint32_t* dst, float* ms) -> int {
+    auto run = [&](int bins, int tw, int waves, uint32_t* dst, float* ms) -> int {
         fd_instance q = *inst;
         q.bins_per_side = bins;
         q.tile_width = tw;
+        q.occ_waves = waves;
         q.d_shadow = (float*)dst;
         q.d_masks = dst + WH;
         fd_batch* b = nullptr;
@@ -910,11 +921,11 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     };
     int rc = hip_status(e);
     float refMs = 0.0f;
-    if (rc == FD_OK) rc = run(16, 8, (uint32_t*)ref.p, &refMs);
+    if (rc == FD_OK) rc = run(16, 8, FD_OCC_WAVES_DEFAULT, (uint32_t*)ref.p, &refMs);
     *best = -1;
     for (int i = 0; i < n && rc == FD_OK; ++i) {
         float ms = 0.0f;
-        rc = run(cands[i].bins_per_side, cands[i].tile_width, (uint32_t*)out.p, &ms);
+        rc = run(cands[i].bins_per_side, cands[i].tile_width, cands[i].occ_waves, (uint32_t*)out.p, &ms);
         if (rc == FD_ERR_CAPACITY) {   // this grid needs more pairs than the instance's arena
             cands[i].ms = -2.0f;
             rc = FD_OK;

@@ -76,17 +76,19 @@ struct FdArgs {
     // order of each bin's triangle list in the per-sample occlusion:
     // FD_TRI_ORDER_AUTO, FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA
     int triOrder;
+    // waves per occlusion work item (per-sample kernel): 2 or 4
+    int occWaves;
 };
 
-// Waves per occlusion work item (per-sample kernel), for 32-bit and 64-bit
-// rows: the waves share a work item's receivers and LDS rows and split its
-// triangle list.
-#ifndef FD_OCC_WAVES
-#define FD_OCC_WAVES 4
-#endif
-#ifndef FD_OCC_WAVES64
-#define FD_OCC_WAVES64 4
-#endif
+// Waves per occlusion work item (per-sample kernel) unless a frame asks for
+// another count: the waves share a work item's receivers and LDS rows and
+// split its triangle list.
+#define FD_OCC_WAVES_DEFAULT 4
+
+// Occlusion classes of a batched launch: 32-bit / 64-bit rows x 4 / 2 waves
+// per work item (one launch each).
+#define FD_OCC_CLASSES 4
+inline int fd_occ_class(int wpr, int occWaves) { return (wpr - 1) + (occWaves == 2 ? 2 : 0); }
 
 
 #define FD_LDS_BINS 1024
@@ -106,17 +108,17 @@ struct FdArgs {
 #define FD_FIN_THREADS 1024   // k_finalize / k_post_pairs block size
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
-enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC32,
-             FD_MAP_OCC64, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO, FD_MAP_NB, FD_NMAPS };
+enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC0,
+             FD_MAP_OCC1, FD_MAP_OCC2, FD_MAP_OCC3, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO, FD_MAP_NB, FD_NMAPS };
 
 struct FdBatchPlan {
     const FdArgs* inst;       // device: instance table (n)
-    const FdArgs* occ[2];     // device: instances of the u32 / u64 occlusion classes, largest first
-    int n, nocc[2];
+    const FdArgs* occ[FD_OCC_CLASSES];   // device: instances of each occlusion class (fd_occ_class), largest first
+    int n, nocc[FD_OCC_CLASSES];
     const int* map[FD_NMAPS]; // device prefix arrays
     int total[FD_NMAPS];      // blocks per stage launch
     int maxNb;                // largest bin count (dynamic LDS sizes)
-    int maxRes[2];            // largest light grid per occlusion class
+    int maxRes[FD_OCC_CLASSES];   // largest light grid per occlusion class
     bool anyTris;
 };
 

@@ -1845,7 +1845,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         const FdTri t = tris[tid];
         // the whole record in one round trip (the prefiltered pairs mostly
         // pass the cull below, which needs only part of it)
-        if (!COUNT) asm volatile("" ::"s"(t.v0[0]), "s"(t.n[0]), "s"(t.e20[0]));
+        if (!COUNT) asm volatile("" ::"s"(t.v0[0]), "s"(t.bminx), "s"(t.minz), "s"(t.n[0]), "s"(t.e20[0]));
         // containment + depth cull (:120-127)
         const bool pass = live && t.bminx <= q.rminx && t.bminy <= q.rminy && q.rmaxx <= t.bmaxx &&
                           q.rmaxy <= t.bmaxy && t.minz <= q.z;
@@ -2179,20 +2179,16 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 __global__ __launch_bounds__(256) void k_zero_masks(FdArgs a) { zero_masks_body(a, blockIdx.x); }
 __global__ __launch_bounds__(64) void k_occlude_finish(FdArgs a) { occlude_finish_body(a, blockIdx.x); }
 
-// Waves per occlusion work item by row width (they share its receivers and
-// LDS rows): the 8 KiB of 32-bit rows of 64 receivers go to 4 waves, the
-// 32 KiB of 64-bit rows to 8, so that LDS never bounds the waves per CU.
-// The counting variant runs one wave per work item: each receiver then meets
-// its triangles in list order with the full-row state of all earlier ones,
-// as in the reference, so its work counters do not depend on the schedule.
-template <typename T, bool COUNT = false>
-constexpr int fd_occ_waves() { return COUNT ? 1 : (Bits<T>::width == 64 ? FD_OCC_WAVES64 : FD_OCC_WAVES); }
-
+// NW waves per occlusion work item (they share its receivers and LDS rows):
+// 4 (the default) or 2 per frame (FdArgs::occWaves).  The counting variant
+// runs one wave per work item: each receiver then meets its triangles in
+// list order with the full-row state of all earlier ones, as in the
+// reference, so its work counters do not depend on the schedule.
 // 8 waves per SIMD (the most a SIMD holds): the target makes the compiler
 // keep the scalar registers within 8 waves' share (else 99-103 SGPRs, 7-8).
-template <typename T, bool COUNT>
-__global__ __launch_bounds__((64 * fd_occ_waves<T, COUNT>())) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
-    occlude_body<T, COUNT, false, fd_occ_waves<T, COUNT>()>(a, xcd_block(blockIdx.x, gridDim.x));
+template <typename T, bool COUNT, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude(FdArgs a) {
+    occlude_body<T, COUNT, false, NW>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 template <typename T>
 __global__ __launch_bounds__(64) void k_occlude_cta(FdArgs a) {
@@ -2243,12 +2239,12 @@ FD_BATCHED(kb_post_pairs, FD_FIN_THREADS, post_pairs_body)
 FD_BATCHED(kb_tri_order, FD_ORDER_THREADS, tri_order_body)
 #undef FD_BATCHED
 
-template <typename T>
-__global__ __launch_bounds__(64 * fd_occ_waves<T>()) __attribute__((amdgpu_waves_per_eu(8))) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(8))) void kb_occlude(const FdArgs* __restrict__ inst, const int* __restrict__ map, int n) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
-    occlude_body<T, false, false, fd_occ_waves<T>()>(a, b - map[i]);
+    occlude_body<T, false, false, NW>(a, b - map[i]);
 }
 __global__ __launch_bounds__(256) void kb_zero_masks(const FdArgs* __restrict__ inst, const int* __restrict__ map,
                                                      int n) {
@@ -2447,11 +2443,16 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.anyTris) kb_tri_order<<<L(FD_MAP_NB), FD_ORDER_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_NB], n);
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < FD_OCC_CLASSES; ++c) {   // fd_occ_class: 32 / 64-bit rows x 4 / 2 waves
         if (p.nocc[c] == 0) continue;
-        const size_t shm = (size_t)(c + 1) * p.maxRes[c] * 64 * sizeof(uint32_t);
-        if (c == 0) kb_occlude<uint32_t><<<L(FD_MAP_OCC32), 64 * fd_occ_waves<uint32_t>(), shm, st>>>(p.occ[0], p.map[FD_MAP_OCC32], p.nocc[0]);
-        else kb_occlude<uint64_t><<<L(FD_MAP_OCC64), 64 * fd_occ_waves<uint64_t>(), shm, st>>>(p.occ[1], p.map[FD_MAP_OCC64], p.nocc[1]);
+        const int wpr = (c & 1) + 1, nw = (c & 2) ? 2 : 4;
+        const size_t shm = (size_t)wpr * p.maxRes[c] * 64 * sizeof(uint32_t);
+        const dim3 g = L(FD_MAP_OCC0 + c);
+        const int* mp = p.map[FD_MAP_OCC0 + c];
+        if (c == 0) kb_occlude<uint32_t, 4><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
+        else if (c == 1) kb_occlude<uint64_t, 4><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
+        else if (c == 2) kb_occlude<uint32_t, 2><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
+        else kb_occlude<uint64_t, 2><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
     }
     if (p.total[FD_MAP_FIN] > 0) kb_occlude_finish<<<L(FD_MAP_FIN), 64, 0, st>>>(p.occ[0], p.map[FD_MAP_FIN], n);
     return hipGetLastError();
@@ -2543,13 +2544,15 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
         const FdArgs& ao = a;
         const int chunks = a.chunkSlots * a.segs;
         const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t);
-        const int threads = 64 * (wpr == 2 ? fd_occ_waves<uint64_t>() : fd_occ_waves<uint32_t>());
         if (a.opCount) {
-            if (wpr == 1) k_occlude<uint32_t, true><<<chunks, 64 * fd_occ_waves<uint32_t, true>(), shm, st>>>(ao);
-            else k_occlude<uint64_t, true><<<chunks, 64 * fd_occ_waves<uint64_t, true>(), shm, st>>>(ao);
+            if (wpr == 1) k_occlude<uint32_t, true, 1><<<chunks, 64, shm, st>>>(ao);
+            else k_occlude<uint64_t, true, 1><<<chunks, 64, shm, st>>>(ao);
+        } else if (a.occWaves == 2) {
+            if (wpr == 1) k_occlude<uint32_t, false, 2><<<chunks, 128, shm, st>>>(ao);
+            else k_occlude<uint64_t, false, 2><<<chunks, 128, shm, st>>>(ao);
         } else {
-            if (wpr == 1) k_occlude<uint32_t, false><<<chunks, threads, shm, st>>>(ao);
-            else k_occlude<uint64_t, false><<<chunks, threads, shm, st>>>(ao);
+            if (wpr == 1) k_occlude<uint32_t, false, 4><<<chunks, 256, shm, st>>>(ao);
+            else k_occlude<uint64_t, false, 4><<<chunks, 256, shm, st>>>(ao);
         }
     }
     if (a.segs > 1) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);

@@ -301,12 +301,13 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     FdLight L;
     light_setup(r->light, L);
     // layout of this request's class: the tuning table, else the defaults
-    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width;
+    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width, waves = FD_OCC_WAVES_DEFAULT;
     for (const fd_tune_entry& t : s->tune)
         if ((t.scene < 0 || t.scene == r->scene) && t.width == r->width && t.height == r->height &&
             t.light_res == r->light.res_w && t.ms >= 0.0f) {
             bins = t.bins_per_side;
             tw = t.tile_width;
+            if (t.occ_waves == 2 || t.occ_waves == 4) waves = t.occ_waves;
             break;
         }
     int g = 0, tl = 0;
@@ -315,6 +316,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     if ((1 << g) != bins || g < 3 || g > 6 || (1 << tl) != tw || tl < 3 || tl > 6) return FD_ERR_ARG;
     fill_args(q.args, md, (const float4*)r->d_position4, (const float4*)r->d_normal4, r->width,
               r->height, L, s->cfg.eps, s->cfg.backface_cull, g, tl, r->d_shadow, r->d_masks, s->cfg.max_pairs);
+    q.args.occWaves = waves;
     FdArgs probe = q.args;
     q.bytes = layout(probe, nullptr, false);
     if (q.bytes > s->pool.size()) return FD_ERR_CAPACITY;   // can never be admitted

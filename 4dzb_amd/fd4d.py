@@ -59,13 +59,13 @@ class Instance(ctypes.Structure):
     _fields_ = [("d_position4", ctypes.c_void_p), ("d_normal4", ctypes.c_void_p), ("width", ctypes.c_int),
                 ("height", ctypes.c_int), ("light", AreaLight), ("d_shadow", ctypes.c_void_p),
                 ("d_masks", ctypes.c_void_p), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
-                ("max_pairs", ctypes.c_ulonglong)]
+                ("max_pairs", ctypes.c_ulonglong), ("occ_waves", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("light_res", ctypes.c_int), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
-                ("ms", ctypes.c_float), ("pad", ctypes.c_int)]
+                ("ms", ctypes.c_float), ("occ_waves", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -131,6 +131,7 @@ def lib():
             "fd_set_tile_width": ([vp, ip], ip),
             "fd_set_occlusion_variant": ([vp, ip], ip),
             "fd_set_triangle_order": ([vp, ip], ip),
+            "fd_set_occlusion_waves": ([vp, ip], ip),
             "fd_autotune": ([vp, ctypes.c_void_p, ctypes.c_void_p, ip, ip, ctypes.POINTER(ip)], ip),
             "fd_set_view": ([vp, ip, ip], ip),
             "fd_set_scenes_esp": ([vp, ctypes.c_float], ip),
@@ -332,6 +333,10 @@ class Context:
         mask rows along y_delt)."""
         _check(lib().fd_set_occlusion_variant(self.h, int(variant)), f"fd_set_occlusion_variant({variant})")
 
+    def set_occlusion_waves(self, waves):
+        """Waves per occlusion work item: 4 (default, 0) or 2 (changes no result)."""
+        _check(lib().fd_set_occlusion_waves(self.h, int(waves)), f"fd_set_occlusion_waves({waves})")
+
     def set_triangle_order(self, order):
         """TRI_ORDER_AUTO / TRI_ORDER_LIST / TRI_ORDER_AREA: the order in which
         the occlusion walks each bin's triangle list (changes no result)."""
@@ -339,15 +344,18 @@ class Context:
 
     def autotune(self, inst_item, candidates, reps=3):
         """Autotuner (fd_autotune): inst_item as a Batch item (its outputs are
-        not written); candidates: list of (bins, tile_width).  Returns
-        (best index or -1, [ms per candidate: -1 = not bit-exact, -2 = its
-        pairs do not fit the instance's pair arena])."""
+        not written); candidates: list of (bins, tile_width) or (bins,
+        tile_width, waves per occlusion work item).  Returns (best index or
+        -1, [ms per candidate: -1 = not bit-exact, -2 = its pairs do not fit
+        the instance's pair arena])."""
         it = inst_item
         inst = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
-                        it["shadow"].ptr, 0, 0, 0, it.get("max_pairs", 0))
+                        it["shadow"].ptr, 0, 0, 0, it.get("max_pairs", 0), 0, 0)
         arr = (TuneEntry * len(candidates))()
-        for k, (b, t) in enumerate(candidates):
-            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, 0)
+        for k, c in enumerate(candidates):
+            b, t = c[0], c[1]
+            w = c[2] if len(c) > 2 else 0
+            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, w)
         best = ctypes.c_int(-1)
         _check(lib().fd_autotune(self.h, ctypes.byref(inst), ctypes.cast(arr, ctypes.c_void_p), len(candidates),
                                  reps, ctypes.byref(best)), "fd_autotune")
@@ -434,6 +442,7 @@ class FrameRunner:
         c.set_scenes_esp(frame.eps)
         c.set_bin_num(1 << frame.grid_log2)
         c.set_tile_width(frame.tile_width)
+        c.set_occlusion_waves(getattr(frame, "occ_waves", 0))
         if pair_capacity:
             c.set_pair_capacity(pair_capacity)
         c.add_model(self.verts, sc.nverts, sc.verts.shape[1] * 4, self.idx, sc.ntris)
@@ -466,13 +475,13 @@ class Batch:
 
     def __init__(self, ctx, items):
         """items: list of dicts with keys pos, nrm (DeviceBuffer), w, h, light
-        (LightDesc), shadow, masks (DeviceBuffer or None), bins and tile
-        (0 = the context's setting), max_pairs."""
+        (LightDesc), shadow, masks (DeviceBuffer or None), bins, tile and
+        waves (0 = the context's setting), max_pairs."""
         arr = (Instance * len(items))()
         for k, it in enumerate(items):
             arr[k] = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
                               it["shadow"].ptr, it["masks"].ptr if it.get("masks") is not None else 0,
-                              it.get("bins", 0), it.get("tile", 0), it.get("max_pairs", 0))
+                              it.get("bins", 0), it.get("tile", 0), it.get("max_pairs", 0), it.get("waves", 0), 0)
         h = ctypes.c_void_p()
         _check(lib().fd_batch_create(ctx.h, ctypes.cast(arr, ctypes.c_void_p), len(items), ctypes.byref(h)),
                "fd_batch_create")
@@ -514,7 +523,7 @@ class Server:
         self._tune = (TuneEntry * max(1, len(tune)))()
         for k, t in enumerate(tune):
             self._tune[k] = TuneEntry(t.get("scene", -1), t["w"], t["h"], t["res"], t["bins"], t["tile"],
-                                      float(t.get("ms", 0.0)), 0)
+                                      float(t.get("ms", 0.0)), t.get("waves", 0))
         cfg = ServerConfig(budget_bytes, streams, max_batch, bins, cull, eps, tile, max_pairs,
                            ctypes.cast(self._tune, ctypes.c_void_p).value if tune else None, len(tune), 0)
         h = ctypes.c_void_p()

@@ -121,6 +121,7 @@ class Frame:
     backface_cull: int = 1
     grid_log2: int = 4
     tile_width: int = 8
+    occ_waves: int = 0      # waves per occlusion work item (0: the library's default)
 
 
 def receivers_plane(w, h, half=4.0):

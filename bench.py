@@ -276,7 +276,7 @@ def gpu_batched(pkg, reps=3):
     cache = at.tune_classes(ctx, 0, items)
     t_tune = time.perf_counter() - t_tune
     for it in items:
-        it["bins"], it["tile"] = at.layout(cache, 0, it["w"], it["h"], it["light"].res_w)
+        it["bins"], it["tile"], it["waves"] = at.layout(cache, 0, it["w"], it["h"], it["light"].res_w)
     # batched, graph-captured
     b = fd.Batch(ctx, items)
     b.capture(strm.handle)
@@ -338,7 +338,8 @@ def gpu_batched(pkg, reps=3):
                 speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact),
                 per_class=per_class,
                 autotune=dict(seconds=round(t_tune, 2), classes={k: dict(bins=v["bins"], tile=v["tile"],
-                                                                        ms=v["ms"], default_ms=v["default_ms"])
+                                                                        waves=v.get("waves", 4), ms=v["ms"],
+                                                                        default_ms=v["default_ms"])
                                                                for k, v in sorted(cache.items())}))
 
 
@@ -478,7 +479,8 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
                                  served_span_ms=round(sus["span"], 1), mean_batch=round(sus["mean_batch"], 2),
                                  failed_requests=sus["failed"], mismatched_requests=sus["mismatched"]),
                 autotune=dict(seconds=round(t_tune, 2), classes=len(cache),
-                              non_default=sum(1 for v in cache.values() if (v["bins"], v["tile"]) != (16, 8))))
+                              non_default=sum(1 for v in cache.values()
+                                              if (v["bins"], v["tile"], v.get("waves", 4)) != (16, 8, 4))))
 
 
 def gpu_target_config(pkg, steps=10):
@@ -494,10 +496,11 @@ def gpu_target_config(pkg, steps=10):
                 max_pairs=64 << 20)
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
-    bins, tile = cands[best] if best >= 0 else (16, 8)
+    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.DEFAULT
     run.close()
     frame.grid_log2 = bins.bit_length() - 1
     frame.tile_width = tile
+    frame.occ_waves = waves
     run = fd.FrameRunner(frame, stream=strm)
     run.run()
     lat = []
@@ -507,7 +510,7 @@ def gpu_target_config(pkg, steps=10):
         lat.append((time.perf_counter() - a) * 1e3)
     run.close()
     return dict(workload="boxes-1080p with a 32x32 light (1920x1080 receivers, 1M triangles)",
-                layout=dict(bins=bins, tile=tile), frames=steps,
+                layout=dict(bins=bins, tile=tile, waves=waves), frames=steps,
                 p50_ms=round(pctl(lat, 50), 2), p99_ms=round(pctl(lat, 99), 2), exit=st.exit_code,
                 gbps=round(1920 * 1080 * 1024 / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
@@ -525,10 +528,11 @@ def gpu_autotuned_layout(pkg, frame, steps=20):
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
     run.close()
-    bins, tile = cands[best] if best >= 0 else (1 << frame.grid_log2, frame.tile_width)
+    bins, tile, waves = cands[best] if best >= 0 else (1 << frame.grid_log2, frame.tile_width, 4)
     f2 = copy.copy(frame)
     f2.grid_log2 = bins.bit_length() - 1
     f2.tile_width = tile
+    f2.occ_waves = waves
     run = fd.FrameRunner(f2, stream=strm)
     run.run()
     for _ in range(3):
@@ -542,7 +546,7 @@ def gpu_autotuned_layout(pkg, frame, steps=20):
     run.close()
     strm.close()
     W, H, R = frame.rx.w, frame.rx.h, frame.light.res_w
-    return dict(layout=dict(bins=bins, tile=tile), ms_per_step=round(ms, 4),
+    return dict(layout=dict(bins=bins, tile=tile, waves=waves), ms_per_step=round(ms, 4),
                 gbps=round(W * H * R * R / (ms * 1e-3) / 1e9, 3))
 
 
@@ -589,10 +593,11 @@ def gpu_shard_tile(pkg, steps=10):
                 max_pairs=64 << 20)
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
-    bins, tile = cands[best] if best >= 0 else (16, 8)
+    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.DEFAULT
     run.close()
     frame.grid_log2 = bins.bit_length() - 1
     frame.tile_width = tile
+    frame.occ_waves = waves
     run = fd.FrameRunner(frame, stream=strm)
     run.run()
     lat = []
@@ -605,7 +610,7 @@ def gpu_shard_tile(pkg, steps=10):
     n = frame.rx.w * frame.rx.h
     R = frame.light.res_w
     return dict(workload="480x540 tile (1/8 of boxes-1080p's 1920x1080) with the 64x64 light, 1M triangles",
-                layout=dict(bins=bins, tile=tile), p50_ms=round(pctl(lat, 50), 2), exit=st.exit_code,
+                layout=dict(bins=bins, tile=tile, waves=waves), p50_ms=round(pctl(lat, 50), 2), exit=st.exit_code,
                 gbps=round(n * R * R / (pctl(lat, 50) * 1e-3) / 1e9, 3))
 
 

@@ -156,6 +156,14 @@ int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
 #define FD_TRI_ORDER_AREA 2
 int fd_set_triangle_order(fd_ctx* ctx, int order);
 
+/* No counterpart (a schedule knob of this implementation): waves per work
+ * item of the per-sample occlusion (64 receivers of one bin, whose triangle
+ * list the waves split): 4 (the default; 0 selects it) or 2.  Results do not
+ * depend on it; frames with many work items can run faster with 2.  The
+ * batched form takes it per instance (fd_instance.occ_waves), the serving
+ * form per class (fd_tune_entry.occ_waves). */
+int fd_set_occlusion_waves(fd_ctx* ctx, int waves);
+
 /* FD::FDsetView, FD/api.cpp:205-237. */
 int fd_set_view(fd_ctx* ctx, int width, int height);
 
@@ -222,6 +230,8 @@ typedef struct fd_instance {
     int bins_per_side;          /* 0 = the context's setting */
     int tile_width;             /* 0 = the context's setting (fd_set_tile_width) */
     unsigned long long max_pairs; /* pair arena of this instance; 0 = 16M */
+    int occ_waves;              /* 0 = the context's setting (fd_set_occlusion_waves) */
+    int pad;
 } fd_instance;
 typedef struct fd_batch fd_batch;
 int fd_batch_create(fd_ctx* ctx, const fd_instance* instances, int n, fd_batch** out);
@@ -249,7 +259,7 @@ typedef struct fd_tune_entry {
     int bins_per_side;
     int tile_width;
     float ms;                 /* measured frame time; < 0: not bit-exact */
-    int pad;
+    int occ_waves;            /* waves per occlusion work item: 2 or 4 (0 = 4) */
 } fd_tune_entry;
 typedef struct fd_server_config {
     size_t hbm_budget_bytes;  /* workspace pool (>= 64 MiB) */
@@ -305,7 +315,8 @@ double fd_server_now_ms(fd_server* srv);
 int fd_server_destroy(fd_server* srv);
 
 /* No counterpart: the autotuner.  Runs `inst` over the context's scene once
- * per candidate layout (cands[i].bins_per_side, cands[i].tile_width), timing
+ * per candidate layout (cands[i].bins_per_side, cands[i].tile_width,
+ * cands[i].occ_waves), timing
  * `reps` launches on the device, and compares every output bit with the
  * default layout (16x16 bins, 8-wide tiles).  cands[i].ms receives the median
  * frame time, -1 when the candidate's shadow values or masks differ (the bin

@@ -65,6 +65,40 @@ def test_batch_equals_alone(fd, wl):
     ctx.close()
 
 
+def test_occlusion_waves_give_identical_bits(fd, wl):
+    """2 or 4 waves per occlusion work item: the same bits, alone (both row
+    widths) and in one batch that mixes them (four occlusion classes)."""
+    scene, ctx, items, keep = _setup(fd, wl)
+    alone = [_alone(fd, wl, scene, it) for it in items]
+    for it in items[:2] + items[3:4]:   # 8, 16 and 33 light samples per side
+        fr = wl.Frame("w2", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1,
+                      grid_log2=(it["bins"].bit_length() - 1) if it["bins"] else 4, occ_waves=2)
+        run = fd.FrameRunner(fr)
+        rc, st = run.run()
+        assert rc == 0
+        sh, mk = run.outputs()
+        run.close()
+        (sh0, mk0), _ = alone[items.index(it)]
+        assert np.array_equal(sh.view(np.uint32), sh0.view(np.uint32)) and np.array_equal(mk, mk0)
+    for k, it in enumerate(items):
+        it["waves"] = 2 if k % 2 else 4
+        it["shadow"].upload(np.full(it["w"] * it["h"], -7.0, np.float32))
+    print("[waves] mixed batch", flush=True)
+    stream = fd.Stream()
+    b = fd.Batch(ctx, items)
+    b.launch(stream.handle)
+    stream.sync()
+    for k, it in enumerate(items):
+        (sh0, mk0), st0 = alone[k]
+        rc, st = b.stats(k)
+        assert rc == 0 and st.exit_code == st0.exit_code, k
+        assert np.array_equal(it["shadow"].download(np.float32).view(np.uint32), sh0.view(np.uint32)), k
+        assert np.array_equal(it["masks"].download(np.uint32), mk0), k
+    b.close()
+    stream.close()
+    ctx.close()
+
+
 def test_batch_instance_against_oracle(fd, wl, oracle):
     scene, ctx, items, keep = _setup(fd, wl)
     b = fd.Batch(ctx, items)
@@ -99,7 +133,7 @@ def test_tile_widths_and_autotune(fd, wl, oracle):
     frame.tile_width = 8
     fr = fd.FrameRunner(frame)
     item = dict(pos=fr.pos, nrm=fr.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=fr.shadow)
-    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64)]
+    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64), (16, 8, 2), (32, 16, 2)]
     best, ms = fr.ctx.autotune(item, cands, reps=2)
     assert 0 <= best < len(cands) and ms[best] > 0
     assert ms[1] > 0   # the reference layout itself is always exact

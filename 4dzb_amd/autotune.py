@@ -4,8 +4,8 @@ For one instance class -- scene, receiver size, light grid -- it runs a
 representative instance under every candidate layout (bins per light-plane
 side x receiver tile width x waves per occlusion work item) through the
 library's fd_autotune, which times each on the device and compares every
-output bit with the default layout (16x16 bins, 8x8 tiles, 4 waves).  The
-fastest bit-exact layout wins.  The bin grid
+output bit with the default layout (16x16 bins, 8x8 tiles, the library's
+default waves).  The fastest bit-exact layout wins.  The bin grid
 also sets the receiver sort's key width (one counting-sort pass over
 2*log2(bins) + 1 bits), so picking the grid picks the sort's radix width.
 
@@ -21,9 +21,18 @@ import time
 
 BINS = (8, 16, 32)
 TILES = (8, 16, 32)
-WAVES = (4, 2)
+WAVES = (4, 8)
 CANDIDATES = [(b, t, w) for w in WAVES for b in BINS for t in TILES]
-DEFAULT = (16, 8, 4)
+
+
+def default_waves(res):
+    """The library's default waves per occlusion work item for a res x res
+    light grid (fd_set_occlusion_waves)."""
+    return 8 if res > 32 else 4
+
+
+def default_layout(res):
+    return (16, 8, default_waves(res))
 
 
 def class_key(scene, w, h, res):
@@ -35,8 +44,9 @@ def tune_one(ctx, item, candidates=CANDIDATES, reps=3):
     nrm, w, h, light, shadow).  Returns the cache entry."""
     t0 = time.perf_counter()
     best, ms = ctx.autotune(item, candidates, reps)
-    default = ms[candidates.index(DEFAULT)] if DEFAULT in candidates else None
-    b, t, w = candidates[best] if best >= 0 else DEFAULT   # nothing bit-exact besides the reference: the default
+    dl = default_layout(item["light"].res_w)
+    default = ms[candidates.index(dl)] if dl in candidates else None
+    b, t, w = candidates[best] if best >= 0 else dl   # nothing bit-exact besides the reference: the default
     return dict(bins=b, tile=t, waves=w, ms=(ms[best] if best >= 0 else default), default_ms=default,
                 candidates={f"{cb}x{ct}x{cw}": (round(m, 4) if m >= 0 else ("not exact" if m == -1 else "pair arena too small"))
                             for (cb, ct, cw), m in zip(candidates, ms)},
@@ -60,7 +70,7 @@ def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3):
 def layout(cache, scene, w, h, res):
     """(bins, tile, waves) of a class, or the default layout."""
     e = cache.get(class_key(scene, w, h, res))
-    return (e["bins"], e["tile"], e.get("waves", DEFAULT[2])) if e else DEFAULT
+    return (e["bins"], e["tile"], e.get("waves", default_waves(res))) if e else default_layout(res)
 
 
 def server_table(cache):
@@ -70,7 +80,7 @@ def server_table(cache):
         scene, wh, res = key.split(":")
         w, h = wh.split("x")
         out.append(dict(scene=int(scene), w=int(w), h=int(h), res=int(res), bins=e["bins"], tile=e["tile"],
-                        waves=e.get("waves", DEFAULT[2]), ms=e["ms"] or 0.0))
+                        waves=e.get("waves", default_waves(int(res))), ms=e["ms"] or 0.0))
     return out
 
 

@@ -92,7 +92,7 @@ struct fd_ctx {
     int tileLog2 = 3;     // 8x8 receiver tiles
     int occVariant = FD_OCC_PER_SAMPLE;
     int triOrder = FD_TRI_ORDER_AUTO;
-    int occWaves = FD_OCC_WAVES_DEFAULT;
+    int occWaves = 0;     // 0: fd_default_waves of the frame's row width
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
     std::vector<Model> models;
@@ -223,7 +223,7 @@ int frame_args(fd_ctx* c, FdArgs& a) {
               c->masks, c->pairCap);
     a.occVariant = c->occVariant;
     a.triOrder = c->triOrder;
-    a.occWaves = c->occWaves;
+    a.occWaves = c->occWaves ? c->occWaves : fd_default_waves(a.L.wpr);
     return FD_OK;
 }
 
@@ -378,8 +378,8 @@ int fd_set_occlusion_variant(fd_ctx* c, int variant) {
 }
 
 int fd_set_occlusion_waves(fd_ctx* c, int waves) {
-    if (!c || (waves != 0 && waves != 2 && waves != 4)) return FD_ERR_ARG;
-    c->occWaves = waves ? waves : FD_OCC_WAVES_DEFAULT;
+    if (!c || (waves != 0 && !fd_valid_waves(waves))) return FD_ERR_ARG;
+    c->occWaves = waves;
     return FD_OK;
 }
 
@@ -795,12 +795,12 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
         }
         FdLight L;
         light_setup(q.light, L);
-        if (q.occ_waves != 0 && q.occ_waves != 2 && q.occ_waves != 4) { delete b; return FD_ERR_ARG; }
+        if (q.occ_waves != 0 && !fd_valid_waves(q.occ_waves)) { delete b; return FD_ERR_ARG; }
         const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
                   c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
         b->args[i].triOrder = c->triOrder;
-        b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves;
+        b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves ? c->occWaves : fd_default_waves(L.wpr);
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
     }
@@ -921,7 +921,7 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     };
     int rc = hip_status(e);
     float refMs = 0.0f;
-    if (rc == FD_OK) rc = run(16, 8, FD_OCC_WAVES_DEFAULT, (uint32_t*)ref.p, &refMs);
+    if (rc == FD_OK) rc = run(16, 8, 0, (uint32_t*)ref.p, &refMs);
     *best = -1;
     for (int i = 0; i < n && rc == FD_OK; ++i) {
         float ms = 0.0f;

@@ -76,19 +76,22 @@ struct FdArgs {
     // order of each bin's triangle list in the per-sample occlusion:
     // FD_TRI_ORDER_AUTO, FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA
     int triOrder;
-    // waves per occlusion work item (per-sample kernel): 2 or 4
+    // waves per occlusion work item (per-sample kernel): 2, 4 or 8
     int occWaves;
 };
 
 // Waves per occlusion work item (per-sample kernel) unless a frame asks for
-// another count: the waves share a work item's receivers and LDS rows and
-// split its triangle list.
-#define FD_OCC_WAVES_DEFAULT 4
+// another count (2, 4 or 8): the waves share a work item's receivers, LDS
+// rows and full-row masks and split its triangle list.  4 for 32-bit rows
+// (8 KiB of rows per work item), 8 for 64-bit rows (32 KiB: LDS holds 4
+// work items per CU, so 8 waves each fill it).
+inline int fd_default_waves(int wpr) { return wpr == 2 ? 8 : 4; }
+inline bool fd_valid_waves(int w) { return w == 2 || w == 4 || w == 8; }
 
-// Occlusion classes of a batched launch: 32-bit / 64-bit rows x 4 / 2 waves
-// per work item (one launch each).
-#define FD_OCC_CLASSES 4
-inline int fd_occ_class(int wpr, int occWaves) { return (wpr - 1) + (occWaves == 2 ? 2 : 0); }
+// Occlusion classes of a batched launch: 32-bit / 64-bit rows x 2 / 4 / 8
+// waves per work item (one launch each).
+#define FD_OCC_CLASSES 6
+inline int fd_occ_class(int wpr, int occWaves) { return (wpr - 1) * 3 + (occWaves == 2 ? 0 : occWaves == 4 ? 1 : 2); }
 
 
 #define FD_LDS_BINS 1024
@@ -109,7 +112,9 @@ inline int fd_occ_class(int wpr, int occWaves) { return (wpr - 1) + (occWaves ==
 
 // Block maps of a batched launch (prefix sums over instances, n + 1 each).
 enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MAP_TRI, FD_MAP_SCANT, FD_MAP_OCC0,
-             FD_MAP_OCC1, FD_MAP_OCC2, FD_MAP_OCC3, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO, FD_MAP_NB, FD_NMAPS };
+             FD_MAP_OCC1, FD_MAP_OCC2, FD_MAP_OCC3, FD_MAP_OCC4, FD_MAP_OCC5, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO,
+             FD_MAP_NB, FD_NMAPS };
+static_assert(FD_MAP_OCC0 + FD_OCC_CLASSES == FD_MAP_ZERO, "one block map per occlusion class");
 
 struct FdBatchPlan {
     const FdArgs* inst;       // device: instance table (n)

@@ -1696,7 +1696,7 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
 // merged with LDS atomics), so a CU holds twice the waves for the same LDS.
 template <typename T, bool COUNT, bool CTA = false, int NW = 1>
 __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
-    extern __shared__ uint32_t lds[];  // wpr * R * 64 row words (no other LDS: it bounds the waves per CU)
+    extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 full-row masks (T)
     const FdState* s = a.st;
     const int status = s->status;
     const int R = a.L.res;                  // rows == columns
@@ -1770,6 +1770,11 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
         lds[r * 64 + lane] = 0u;
         if (wpr == 2) lds[(R + r) * 64 + lane] = 0u;
     }
+    // rows shared by several waves: the rows each receiver has full, as any
+    // wave found them (after the rows)
+    constexpr bool SHARE_FULL = NW > 1 && !CTA;
+    T* const fullSh = (T*)(lds + wpr * R * 64);
+    if (SHARE_FULL && wv == 0) fullSh[lane] = (T)0;
     if (NW > 1) __syncthreads();
     if (CTA) {
         T* fullRows = (T*)(lds + wpr * R * 64);
@@ -1842,6 +1847,10 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     // one pair (triangle tid, this wave's receivers): cull, planes, rows
     auto step = [&](const int tid) {
         if ((unsigned)tid >= (unsigned)a.ntris) return;   // defensive: pair entries are triangle ids
+        if (SHARE_FULL) {   // rows the other waves filled
+            full |= (T)fullSh[lane];
+            if (full == FULL) live = false;
+        }
         const FdTri t = tris[tid];
         // the whole record in one round trip (the prefiltered pairs mostly
         // pass the cull below, which needs only part of it)
@@ -1965,7 +1974,13 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
                     }
                     cur = (T)(((uint64_t)hi32 << 32) | lo32);
                 }
-                if (cur == FULL) full |= (T)1 << r;
+                if (cur == FULL) {
+                    full |= (T)1 << r;
+                    if (SHARE_FULL) {
+                        if (Bits<T>::width == 32) atomicOr((uint32_t*)(fullSh + lane), (uint32_t)1 << r);
+                        else atomicOr((unsigned long long*)(fullSh + lane), 1ull << r);
+                    }
+                }
                 valid &= valid - (T)1;
             }
         };
@@ -2443,16 +2458,22 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     kb_tables<<<L(FD_MAP_ONE), 1024, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.anyTris) kb_tri_order<<<L(FD_MAP_NB), FD_ORDER_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_NB], n);
     if (p.total[FD_MAP_ZERO] > 0) kb_zero_masks<<<L(FD_MAP_ZERO), 256, 0, st>>>(p.occ[0], p.map[FD_MAP_ZERO], n);
-    for (int c = 0; c < FD_OCC_CLASSES; ++c) {   // fd_occ_class: 32 / 64-bit rows x 4 / 2 waves
+    for (int c = 0; c < FD_OCC_CLASSES; ++c) {   // fd_occ_class: 32 / 64-bit rows x 2 / 4 / 8 waves
         if (p.nocc[c] == 0) continue;
-        const int wpr = (c & 1) + 1, nw = (c & 2) ? 2 : 4;
-        const size_t shm = (size_t)wpr * p.maxRes[c] * 64 * sizeof(uint32_t);
+        const int wpr = c / 3 + 1, nw = 2 << (c % 3);
+        const size_t shm = (size_t)wpr * p.maxRes[c] * 64 * sizeof(uint32_t) + 64 * wpr * sizeof(uint32_t);
         const dim3 g = L(FD_MAP_OCC0 + c);
         const int* mp = p.map[FD_MAP_OCC0 + c];
-        if (c == 0) kb_occlude<uint32_t, 4><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
-        else if (c == 1) kb_occlude<uint64_t, 4><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
-        else if (c == 2) kb_occlude<uint32_t, 2><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
-        else kb_occlude<uint64_t, 2><<<g, 64 * nw, shm, st>>>(p.occ[c], mp, p.nocc[c]);
+        const FdArgs* in = p.occ[c];
+        const int k = p.nocc[c];
+        switch (c) {
+            case 0: kb_occlude<uint32_t, 2><<<g, 64 * nw, shm, st>>>(in, mp, k); break;
+            case 1: kb_occlude<uint32_t, 4><<<g, 64 * nw, shm, st>>>(in, mp, k); break;
+            case 2: kb_occlude<uint32_t, 8><<<g, 64 * nw, shm, st>>>(in, mp, k); break;
+            case 3: kb_occlude<uint64_t, 2><<<g, 64 * nw, shm, st>>>(in, mp, k); break;
+            case 4: kb_occlude<uint64_t, 4><<<g, 64 * nw, shm, st>>>(in, mp, k); break;
+            default: kb_occlude<uint64_t, 8><<<g, 64 * nw, shm, st>>>(in, mp, k); break;
+        }
     }
     if (p.total[FD_MAP_FIN] > 0) kb_occlude_finish<<<L(FD_MAP_FIN), 64, 0, st>>>(p.occ[0], p.map[FD_MAP_FIN], n);
     return hipGetLastError();
@@ -2543,16 +2564,21 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     } else {
         const FdArgs& ao = a;
         const int chunks = a.chunkSlots * a.segs;
-        const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t);
+        const size_t shm = (size_t)wpr * a.L.res * 64 * sizeof(uint32_t) + 64 * wpr * sizeof(uint32_t);
         if (a.opCount) {
             if (wpr == 1) k_occlude<uint32_t, true, 1><<<chunks, 64, shm, st>>>(ao);
             else k_occlude<uint64_t, true, 1><<<chunks, 64, shm, st>>>(ao);
-        } else if (a.occWaves == 2) {
-            if (wpr == 1) k_occlude<uint32_t, false, 2><<<chunks, 128, shm, st>>>(ao);
-            else k_occlude<uint64_t, false, 2><<<chunks, 128, shm, st>>>(ao);
         } else {
-            if (wpr == 1) k_occlude<uint32_t, false, 4><<<chunks, 256, shm, st>>>(ao);
-            else k_occlude<uint64_t, false, 4><<<chunks, 256, shm, st>>>(ao);
+            const int nw = fd_valid_waves(a.occWaves) ? a.occWaves : fd_default_waves(wpr);
+            const int c = fd_occ_class(wpr, nw);
+            switch (c) {
+                case 0: k_occlude<uint32_t, false, 2><<<chunks, 64 * nw, shm, st>>>(ao); break;
+                case 1: k_occlude<uint32_t, false, 4><<<chunks, 64 * nw, shm, st>>>(ao); break;
+                case 2: k_occlude<uint32_t, false, 8><<<chunks, 64 * nw, shm, st>>>(ao); break;
+                case 3: k_occlude<uint64_t, false, 2><<<chunks, 64 * nw, shm, st>>>(ao); break;
+                case 4: k_occlude<uint64_t, false, 4><<<chunks, 64 * nw, shm, st>>>(ao); break;
+                default: k_occlude<uint64_t, false, 8><<<chunks, 64 * nw, shm, st>>>(ao); break;
+            }
         }
     }
     if (a.segs > 1) k_occlude_finish<<<a.chunkSlots, 64, 0, st>>>(a);

@@ -301,13 +301,13 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     FdLight L;
     light_setup(r->light, L);
     // layout of this request's class: the tuning table, else the defaults
-    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width, waves = FD_OCC_WAVES_DEFAULT;
+    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width, waves = fd_default_waves(L.wpr);
     for (const fd_tune_entry& t : s->tune)
         if ((t.scene < 0 || t.scene == r->scene) && t.width == r->width && t.height == r->height &&
             t.light_res == r->light.res_w && t.ms >= 0.0f) {
             bins = t.bins_per_side;
             tw = t.tile_width;
-            if (t.occ_waves == 2 || t.occ_waves == 4) waves = t.occ_waves;
+            if (fd_valid_waves(t.occ_waves)) waves = t.occ_waves;
             break;
         }
     int g = 0, tl = 0;

@@ -334,7 +334,8 @@ class Context:
         _check(lib().fd_set_occlusion_variant(self.h, int(variant)), f"fd_set_occlusion_variant({variant})")
 
     def set_occlusion_waves(self, waves):
-        """Waves per occlusion work item: 4 (default, 0) or 2 (changes no result)."""
+        """Waves per occlusion work item: 2, 4 or 8; 0 = the default (4 for light
+        grids up to 32x32, 8 above).  Changes no result."""
         _check(lib().fd_set_occlusion_waves(self.h, int(waves)), f"fd_set_occlusion_waves({waves})")
 
     def set_triangle_order(self, order):

@@ -193,7 +193,9 @@ def plain_leg(pkg, frame, workload, steps, warmup, quick):
     leg = dict(gbps=round(value, 4), ms_per_step=round(g["ms_per_step"], 4),
                config=dict(workload=workload, receivers=f"{W}x{H}", light_samples=f"{R}x{R}",
                            triangles=frame.scene.ntris, bins=f"{1 << frame.grid_log2}x{1 << frame.grid_log2}",
-                           tile_width=frame.tile_width, path="plain per-frame"),
+                           tile_width=frame.tile_width,
+                           occ_waves=frame.occ_waves or pkg.autotune.default_waves(frame.light.res_w),
+                           path="plain per-frame"),
                p50_ms=round(pctl(g["lat"], 50), 3), p99_ms=round(pctl(g["lat"], 99), 3),
                stage_ms=dict(zip(["light_plane", "receiver_binning", "triangle_pairing", "tables", "occlusion"],
                                  [round(x, 4) for x in g["stage"]])),
@@ -338,7 +340,7 @@ def gpu_batched(pkg, reps=3):
                 speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact),
                 per_class=per_class,
                 autotune=dict(seconds=round(t_tune, 2), classes={k: dict(bins=v["bins"], tile=v["tile"],
-                                                                        waves=v.get("waves", 4), ms=v["ms"],
+                                                                        waves=v.get("waves"), ms=v["ms"],
                                                                         default_ms=v["default_ms"])
                                                                for k, v in sorted(cache.items())}))
 
@@ -479,8 +481,9 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
                                  served_span_ms=round(sus["span"], 1), mean_batch=round(sus["mean_batch"], 2),
                                  failed_requests=sus["failed"], mismatched_requests=sus["mismatched"]),
                 autotune=dict(seconds=round(t_tune, 2), classes=len(cache),
-                              non_default=sum(1 for v in cache.values()
-                                              if (v["bins"], v["tile"], v.get("waves", 4)) != (16, 8, 4))))
+                              non_default=sum(1 for k, v in cache.items()
+                                              if (v["bins"], v["tile"], v.get("waves"))
+                                              != pkg.autotune.default_layout(int(k.split(":")[2])))))
 
 
 def gpu_target_config(pkg, steps=10):
@@ -496,7 +499,7 @@ def gpu_target_config(pkg, steps=10):
                 max_pairs=64 << 20)
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
-    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.DEFAULT
+    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.default_layout(frame.light.res_w)
     run.close()
     frame.grid_log2 = bins.bit_length() - 1
     frame.tile_width = tile
@@ -528,7 +531,7 @@ def gpu_autotuned_layout(pkg, frame, steps=20):
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
     run.close()
-    bins, tile, waves = cands[best] if best >= 0 else (1 << frame.grid_log2, frame.tile_width, 4)
+    bins, tile, waves = cands[best] if best >= 0 else (1 << frame.grid_log2, frame.tile_width, frame.occ_waves)
     f2 = copy.copy(frame)
     f2.grid_log2 = bins.bit_length() - 1
     f2.tile_width = tile
@@ -593,7 +596,7 @@ def gpu_shard_tile(pkg, steps=10):
                 max_pairs=64 << 20)
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
-    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.DEFAULT
+    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.default_layout(frame.light.res_w)
     run.close()
     frame.grid_log2 = bins.bit_length() - 1
     frame.tile_width = tile
@@ -773,6 +776,8 @@ def main():
     ap.add_argument("--light-res", type=int, default=0, help="override the workload's light grid side")
     ap.add_argument("--bins", type=int, default=0, help="bins per light-plane side (default: the workload's, 16)")
     ap.add_argument("--tile", type=int, default=0, help="receiver tile width (default 8)")
+    ap.add_argument("--waves", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="waves per occlusion work item (default: 4 up to 32x32 light samples, 8 above)")
     ap.add_argument("--leg", choices=["batched", "served", "target", "app_pipeline", "shard_tile"], default=None,
                     help="run only this leg and print its JSON (for profiler runs)")
     args = ap.parse_args()
@@ -798,6 +803,8 @@ def main():
         frame.grid_log2 = args.bins.bit_length() - 1
     if args.tile:
         frame.tile_width = args.tile
+    if args.waves:
+        frame.occ_waves = args.waves
     log("[bench] GPU plain path")
     g, leg = plain_leg(pkg, frame, args.workload, args.steps, args.warmup, args.quick)
     out = dict(metric=METRIC, value=leg["gbps"], unit="Gbit/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -806,7 +813,7 @@ def main():
                config=leg["config"], p50_ms=leg["p50_ms"], p99_ms=leg["p99_ms"], stage_ms=leg["stage_ms"],
                frame_stats=leg["frame_stats"], roofline=leg["roofline"])
     if not args.quick and args.workload != "heightfield-1k" and not args.light_res and not args.bins \
-            and not args.tile:
+            and not args.tile and not args.waves:
         log("[bench] GPU plain path, heightfield-1k")
         f2 = make_frame(wl, "heightfield-1k", 0)
         g2, leg2 = plain_leg(pkg, f2, "heightfield-1k", args.steps, args.warmup, False)
@@ -815,7 +822,7 @@ def main():
         log("[bench] bin-queue occlusion variant (heightfield-1k)")
         out["bin_queue_variant"] = dict(gpu_bin_queue(pkg, g2, f2), workload="heightfield-1k")
         g2["run"].close()
-    if not args.quick and not args.bins and not args.tile:
+    if not args.quick and not args.bins and not args.tile and not args.waves:
         log("[bench] autotuned layout (same frame)")
         out["autotuned_layout"] = gpu_autotuned_layout(pkg, frame)
     if not args.quick and args.workload == "heightfield-1k":

@@ -158,8 +158,8 @@ int fd_set_triangle_order(fd_ctx* ctx, int order);
 
 /* No counterpart (a schedule knob of this implementation): waves per work
  * item of the per-sample occlusion (64 receivers of one bin, whose triangle
- * list the waves split): 4 (the default; 0 selects it) or 2.  Results do not
- * depend on it; frames with many work items can run faster with 2.  The
+ * list the waves split): 2, 4 or 8; 0 (the default) selects 4 for light
+ * grids up to 32 x 32 and 8 above.  Results do not depend on it.  The
  * batched form takes it per instance (fd_instance.occ_waves), the serving
  * form per class (fd_tune_entry.occ_waves). */
 int fd_set_occlusion_waves(fd_ctx* ctx, int waves);
@@ -259,7 +259,7 @@ typedef struct fd_tune_entry {
     int bins_per_side;
     int tile_width;
     float ms;                 /* measured frame time; < 0: not bit-exact */
-    int occ_waves;            /* waves per occlusion work item: 2 or 4 (0 = 4) */
+    int occ_waves;            /* waves per occlusion work item: 2, 4 or 8 (0 = default) */
 } fd_tune_entry;
 typedef struct fd_server_config {
     size_t hbm_budget_bytes;  /* workspace pool (>= 64 MiB) */

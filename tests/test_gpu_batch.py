@@ -66,35 +66,40 @@ def test_batch_equals_alone(fd, wl):
 
 
 def test_occlusion_waves_give_identical_bits(fd, wl):
-    """2 or 4 waves per occlusion work item: the same bits, alone (both row
-    widths) and in one batch that mixes them (four occlusion classes)."""
+    """2, 4 or 8 waves per occlusion work item: the same bits, alone (both
+    row widths) and in one batch that mixes them (six occlusion classes)."""
     scene, ctx, items, keep = _setup(fd, wl)
     alone = [_alone(fd, wl, scene, it) for it in items]
     for it in items[:2] + items[3:4]:   # 8, 16 and 33 light samples per side
-        fr = wl.Frame("w2", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1,
-                      grid_log2=(it["bins"].bit_length() - 1) if it["bins"] else 4, occ_waves=2)
-        run = fd.FrameRunner(fr)
-        rc, st = run.run()
-        assert rc == 0
-        sh, mk = run.outputs()
-        run.close()
-        (sh0, mk0), _ = alone[items.index(it)]
-        assert np.array_equal(sh.view(np.uint32), sh0.view(np.uint32)) and np.array_equal(mk, mk0)
-    for k, it in enumerate(items):
-        it["waves"] = 2 if k % 2 else 4
-        it["shadow"].upload(np.full(it["w"] * it["h"], -7.0, np.float32))
-    print("[waves] mixed batch", flush=True)
+        for w in (2, 4, 8):
+            fr = wl.Frame("w", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1,
+                          grid_log2=(it["bins"].bit_length() - 1) if it["bins"] else 4, occ_waves=w)
+            run = fd.FrameRunner(fr)
+            rc, st = run.run()
+            assert rc == 0
+            sh, mk = run.outputs()
+            run.close()
+            (sh0, mk0), _ = alone[items.index(it)]
+            assert np.array_equal(sh.view(np.uint32), sh0.view(np.uint32)) and np.array_equal(mk, mk0), w
+    # two assignments that together put instances in all six classes (the
+    # 33 and 64 light-sample instances are items 3 and 5)
     stream = fd.Stream()
-    b = fd.Batch(ctx, items)
-    b.launch(stream.handle)
-    stream.sync()
-    for k, it in enumerate(items):
-        (sh0, mk0), st0 = alone[k]
-        rc, st = b.stats(k)
-        assert rc == 0 and st.exit_code == st0.exit_code, k
-        assert np.array_equal(it["shadow"].download(np.float32).view(np.uint32), sh0.view(np.uint32)), k
-        assert np.array_equal(it["masks"].download(np.uint32), mk0), k
-    b.close()
+    for waves in [(2, 4, 8, 2, 4, 8, 4), (8, 2, 4, 4, 8, 2, 2)]:
+        print(f"[waves] mixed batch {waves}", flush=True)
+        for k, it in enumerate(items):
+            it["waves"] = waves[k]
+            it["shadow"].upload(np.full(it["w"] * it["h"], -7.0, np.float32))
+            it["masks"].upload(np.full(it["masks"].nbytes // 4, 0xA5A5A5A5, np.uint32))
+        b = fd.Batch(ctx, items)
+        b.launch(stream.handle)
+        stream.sync()
+        for k, it in enumerate(items):
+            (sh0, mk0), st0 = alone[k]
+            rc, st = b.stats(k)
+            assert rc == 0 and st.exit_code == st0.exit_code, (waves, k)
+            assert np.array_equal(it["shadow"].download(np.float32).view(np.uint32), sh0.view(np.uint32)), (waves, k)
+            assert np.array_equal(it["masks"].download(np.uint32), mk0), (waves, k)
+        b.close()
     stream.close()
     ctx.close()
 

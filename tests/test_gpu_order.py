@@ -47,9 +47,16 @@ def _run(fd, frame, order):
     return fr
 
 
-@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("mesh", LIST)])
+def _soup_small(wl):
+    # few enough work items that the frame splits its bins' lists into
+    # stretches (merged by atomic OR), walked in area order
+    return wl.boxes_1080p(w=160, h=90, ntri=60_000, light_res=40)
+
+
+@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", LIST)])
 def test_auto_choice_and_oracle(fd, wl, oracle, which, expect):
-    frame = {"soup": lambda: _soup(wl), "soup32": lambda: _soup(wl, 32), "mesh": lambda: _mesh(wl)}[which]()
+    frame = {"soup": lambda: _soup(wl), "soup32": lambda: _soup(wl, 32), "soup-small": lambda: _soup_small(wl),
+             "mesh": lambda: _mesh(wl)}[which]()
     print(f"[order] auto {which}", flush=True)
     fr = _run(fd, frame, AUTO)
     assert int(fd.read_state(fr.ctx)["triOrder"]) == expect
@@ -146,6 +153,7 @@ def test_batch_over_soup(fd, wl):
         run.ctx.set_triangle_order(AUTO)
         rc, st = run.run()
         areas += int(fd.read_state(run.ctx)["triOrder"]) == AREA
+        assert rc == 0 and _digest(*run.outputs()) == alone[-1]   # split lists in area order: the same bits
         run.close()
     assert areas == len(items)   # the automatic choice takes the area order on every instance
     print("[order] batch over soup", flush=True)

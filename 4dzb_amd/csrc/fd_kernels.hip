@@ -307,12 +307,18 @@ __device__ __forceinline__ int tile_id(int p, int W, int H, int twl) {
 }
 
 __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
-    extern __shared__ uint32_t hist[];  // nb + 1
+    extern __shared__ uint32_t hist[];  // nb + 1 counters, then (grids up to FD_LDS_BINS) 5 x nb bin statistics
     __shared__ uint32_t blkCount[2];    // valid receivers, keys out of range
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
+    // bin statistics of the block in LDS, merged into the global ones once per
+    // bin the block touched (else one set of global atomics per wave and key)
+    const bool ldsStat = a.binLds && nb <= FD_LDS_BINS;
+    uint32_t* bstat = hist + nb + 1;
     for (int k = threadIdx.x; k <= nb; k += blockDim.x) hist[k] = 0;
+    if (ldsStat)
+        for (int k = threadIdx.x; k < 5 * nb; k += blockDim.x) bstat[k] = (k % 5) < 3 ? 0u : 0xffffffffu;
     if (threadIdx.x < 2) blkCount[threadIdx.x] = 0;
     __syncthreads();
     int nvalid = 0, noor = 0;
@@ -358,7 +364,7 @@ __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
                 const uint32_t r3 = wave_min_u32(mine ? v3 : 0xffffffffu);
                 const uint32_t r4 = wave_min_u32(mine ? v4 : 0xffffffffu);
                 if (lane_id() == 0) {
-                    uint32_t* bs = a.binStat + 5 * k0;
+                    uint32_t* bs = (ldsStat ? bstat : a.binStat) + 5 * k0;
                     atomicMax(bs + 0, r0); atomicMax(bs + 1, r1); atomicMax(bs + 2, r2);
                     atomicMin(bs + 3, r3); atomicMin(bs + 4, r4);
                 }
@@ -376,7 +382,16 @@ __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
     // keys, totalled in k_post_pairs; only the rare out-of-range count is
     // accumulated here
     if (threadIdx.x == 0 && blkCount[1]) atomicAdd(&a.st->keyOutOfRange, (int)blkCount[1]);
-    for (int k = threadIdx.x; k <= nb; k += blockDim.x) a.histR[(size_t)k * a.nblkR + blk] = hist[k];
+    for (int k = threadIdx.x; k <= nb; k += blockDim.x) {
+        const uint32_t h = hist[k];
+        a.histR[(size_t)k * a.nblkR + blk] = h;
+        if (ldsStat && k < nb && h) {
+            const uint32_t* l = bstat + 5 * k;
+            uint32_t* bs = a.binStat + 5 * k;
+            atomicMax(bs + 0, l[0]); atomicMax(bs + 1, l[1]); atomicMax(bs + 2, l[2]);
+            atomicMin(bs + 3, l[3]); atomicMin(bs + 4, l[4]);
+        }
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -1071,7 +1086,9 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
 // reference-order list by area class, into the same range of pairKT (free
 // once the pairs are scattered).  One block of FD_ORDER_THREADS per bin:
 // pass 1 counts the classes, pass 2 ranks every entry by wave ballots plus an
-// exclusive prefix over the waves of each slice.
+// exclusive prefix over the waves of each slice.  (Holding a bin's entries
+// in registers for one load phase was slower: the longest lists, which bound
+// the launch, still need two passes.)
 __device__ __forceinline__ void tri_order_body(const FdArgs& a, const int bin) {
     constexpr int NWV = FD_ORDER_THREADS / 64, C = FD_TRI_CLASSES;
     __shared__ int waveCount[NWV][C];
@@ -2436,7 +2453,7 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     kb_init_bins<<<L(FD_MAP_BINS), 256, 0, st>>>(p.inst, p.map[FD_MAP_BINS], n);
     kb_reduce<<<L(FD_MAP_RED), RED_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_RED], n);
     kb_finalize<<<L(FD_MAP_ONE), FD_FIN_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
-    kb_rx_key<<<L(FD_MAP_RX), 256, (nb + 1) * sizeof(uint32_t), st>>>(p.inst, p.map[FD_MAP_RX], n);
+    kb_rx_key<<<L(FD_MAP_RX), 256, (nb + 1) * sizeof(uint32_t) + binLds, st>>>(p.inst, p.map[FD_MAP_RX], n);
     kb_scan_reduce<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
     kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 0);
     kb_scan_apply<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
@@ -2530,7 +2547,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
     k_finalize<<<1, FD_FIN_THREADS, 0, st>>>(a);
     mark(1);
     // receivers: key + bin statistics, stable counting sort
-    k_rx_key<<<a.nblkR, 256, (nb + 1) * sizeof(uint32_t), st>>>(a);
+    k_rx_key<<<a.nblkR, 256, (nb + 1) * sizeof(uint32_t) + binLds, st>>>(a);
     hipError_t e = scan_u32(a.histR, a.nR, a.scanPart, &s->rxTotal, &s->status, st);
     if (e != hipSuccess) return e;
     k_rx_scatter<<<a.nblkR, 64, (nb + 1) * sizeof(int), st>>>(a);

@@ -396,7 +396,8 @@ __device__ __forceinline__ void rx_key_body(const FdArgs& a, const int blk) {
 
 // --------------------------------------------------------------------------
 // Device-wide exclusive scan of uint32 counts (thrust::exclusive_scan /
-// the offsets of a stable counting sort).  Totals are kept in 64 bits.
+// the offsets of a stable counting sort) in two launches: chunk sums, then
+// each chunk's scan offset by the sums before it.  Totals are kept in 64 bits.
 // --------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* tot) {
     __shared__ uint64_t wsum[SCAN_BLOCK / 64];
@@ -432,24 +433,23 @@ __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ d,
     if (threadIdx.x == 0) part[blk] = tot;
 }
 
-__device__ __forceinline__ void scan_partials_body(uint64_t* __restrict__ part, int np, unsigned long long* total,
-                                                   const int* status) {
-    if (status && *status != FD_EXIT_OK) return;
-    uint64_t carry = 0;
-    for (int b0 = 0; b0 < np; b0 += SCAN_BLOCK) {
-        const int i = b0 + threadIdx.x;
-        const uint64_t v = i < np ? part[i] : 0;
-        uint64_t tot;
-        const uint64_t ex = block_excl_scan_u64(v, &tot);
-        if (i < np) part[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
+// Second pass: each chunk adds the sum of the chunk sums before it (read
+// from part[], at most a few hundred values, by the whole block) to its own
+// exclusive scan; block 0 also writes the grand total.
 __device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ d, long n, const uint64_t* __restrict__ part,
-                                                const int* status, const int blk) {
+                                                unsigned long long* total, const int* status, const int blk) {
     if (status && *status != FD_EXIT_OK) return;
+    const int np = (int)((n + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    uint64_t pre = 0, all = 0;
+    for (int i = threadIdx.x; i < np; i += SCAN_BLOCK) {
+        const uint64_t x = part[i];
+        if (i < blk) pre += x;
+        all += x;
+    }
+    uint64_t preTot, allTot;
+    block_excl_scan_u64(pre, &preTot);
+    block_excl_scan_u64(all, &allTot);
+    if (blk == 0 && threadIdx.x == 0 && total) *total = allTot;
     const long base = (long)blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint64_t sum = 0;
@@ -459,7 +459,7 @@ __device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ d, long n
         sum += v[k];
     }
     uint64_t tot;
-    uint64_t off = block_excl_scan_u64(sum, &tot) + part[blk];
+    uint64_t off = block_excl_scan_u64(sum, &tot) + preTot;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         if (base + k < n) d[base + k] = (uint32_t)off;
@@ -2230,13 +2230,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __re
                                                             uint64_t* __restrict__ part, const int* status) {
     scan_reduce_body(d, n, part, status, blockIdx.x);
 }
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_partials(uint64_t* __restrict__ part, int np,
-                                                              unsigned long long* total, const int* status) {
-    scan_partials_body(part, np, total, status);
-}
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict__ d, long n,
-                                                           const uint64_t* __restrict__ part, const int* status) {
-    scan_apply_body(d, n, part, status, blockIdx.x);
+                                                           const uint64_t* __restrict__ part,
+                                                           unsigned long long* total, const int* status) {
+    scan_apply_body(d, n, part, total, status, blockIdx.x);
 }
 
 // instance of block b: map[i] <= b < map[i+1] (wave-uniform binary search)
@@ -2307,18 +2304,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_reduce(const FdArgs* __res
     const FdArgs a = inst[i];
     scan_reduce_body(scan_data(a, which), scan_len(a, which), a.scanPart, &a.st->status, b - map[i]);
 }
-__global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_partials(const FdArgs* __restrict__ inst, int which) {
-    const FdArgs a = inst[blockIdx.x];
-    const long len = scan_len(a, which);
-    scan_partials_body(a.scanPart, (int)((len + SCAN_CHUNK - 1) / SCAN_CHUNK), which ? &a.st->pairs : &a.st->rxTotal,
-                       &a.st->status);
-}
 __global__ __launch_bounds__(SCAN_BLOCK) void kb_scan_apply(const FdArgs* __restrict__ inst, const int* __restrict__ map,
                                                             int n, int which) {
     const int b = blockIdx.x;
     const int i = inst_of(map, n, b);
     const FdArgs a = inst[i];
-    scan_apply_body(scan_data(a, which), scan_len(a, which), a.scanPart, &a.st->status, b - map[i]);
+    scan_apply_body(scan_data(a, which), scan_len(a, which), a.scanPart, which ? &a.st->pairs : &a.st->rxTotal,
+                    &a.st->status, b - map[i]);
 }
 
 // Verification of the row-mask primitive on given (D, B) pairs (n = R - 1),
@@ -2434,8 +2426,7 @@ static hipError_t scan_u32(uint32_t* d, long n, uint64_t* part, unsigned long lo
                            hipStream_t st) {
     const int nch = cdiv(n, SCAN_CHUNK);
     k_scan_reduce<<<nch, SCAN_BLOCK, 0, st>>>(d, n, part, status);
-    k_scan_partials<<<1, SCAN_BLOCK, 0, st>>>(part, nch, total, status);
-    k_scan_apply<<<nch, SCAN_BLOCK, 0, st>>>(d, n, part, status);
+    k_scan_apply<<<nch, SCAN_BLOCK, 0, st>>>(d, n, part, total, status);
     return hipGetLastError();
 }
 
@@ -2455,19 +2446,16 @@ hipError_t enqueue_batch(const FdBatchPlan& p, hipStream_t st) {
     kb_finalize<<<L(FD_MAP_ONE), FD_FIN_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     kb_rx_key<<<L(FD_MAP_RX), 256, (nb + 1) * sizeof(uint32_t) + binLds, st>>>(p.inst, p.map[FD_MAP_RX], n);
     kb_scan_reduce<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
-    kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 0);
     kb_scan_apply<<<L(FD_MAP_SCANR), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANR], n, 0);
     kb_rx_scatter<<<L(FD_MAP_RX), 64, (nb + 1) * sizeof(int), st>>>(p.inst, p.map[FD_MAP_RX], n);
     if (p.anyTris) {
         kb_tri_count<<<L(FD_MAP_TRI), FD_TRB, nb * sizeof(uint32_t) + binLds, st>>>(p.inst, p.map[FD_MAP_TRI], n);
         kb_scan_reduce<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
-        kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 1);
         kb_scan_apply<<<L(FD_MAP_SCANT), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANT], n, 1);
     }
     kb_post_pairs<<<L(FD_MAP_ONE), FD_FIN_THREADS, 0, st>>>(p.inst, p.map[FD_MAP_ONE], n);
     if (p.anyTris) {
         kb_scan_reduce<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
-        kb_scan_partials<<<n, SCAN_BLOCK, 0, st>>>(p.inst, 2);
         kb_scan_apply<<<L(FD_MAP_SCANO), SCAN_BLOCK, 0, st>>>(p.inst, p.map[FD_MAP_SCANO], n, 2);
         kb_tri_write<<<L(FD_MAP_TRI), FD_TRB, binLds, st>>>(p.inst, p.map[FD_MAP_TRI], n);
         kb_tri_scatter_wide<<<L(FD_MAP_TRI), FD_TRB, (size_t)nb * FD_SCATTER_WIDE_LDS, st>>>(p.inst, p.map[FD_MAP_TRI], n);

@@ -1081,76 +1081,108 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
     }
 }
 
+// Walk key of a (bin, triangle) pair for the area order: first how much of
+// the bin's light-plane cell the triangle's culling box covers (eighths,
+// rounded down; a box covering the whole cell first), then the box's area
+// class.  Both are read from the culling box in the float expressions the
+// tests restate: the box in bin units is (b - B) * factor, as in the pair
+// count's clamp.
+__device__ __forceinline__ int walk_key(const float4 mn, const float4 mx, const FdState* s, int bi, int bj) {
+    const float ux0 = (mn.x - s->B[0]) * s->factor[0], ux1 = (mx.x - s->B[0]) * s->factor[0];
+    const float uy0 = (mn.y - s->B[1]) * s->factor[1], uy1 = (mx.y - s->B[1]) * s->factor[1];
+    const float ox = fmaxf(fminf(ux1, (float)(bi + 1)) - fmaxf(ux0, (float)bi), 0.0f);
+    const float oy = fmaxf(fminf(uy1, (float)(bj + 1)) - fmaxf(uy0, (float)bj), 0.0f);
+    const float cov = ox * oy;   // NaN for a NaN box: class 0 below, then the last coverage class
+    const int cc = cov >= 0.0f ? min((int)(cov * 8.0f), FD_COVER_CLASSES - 1) : 0;
+    return (FD_COVER_CLASSES - 1 - cc) * FD_TRI_CLASSES + area_class(mn, mx, s);
+}
+
 // Area order of one bin's triangle list (frames whose walk order is
 // FD_TRI_ORDER_AREA): a stable counting sort of triStart..triEnd of the
-// reference-order list by area class, into the same range of pairKT (free
-// once the pairs are scattered).  One block of FD_ORDER_THREADS per bin:
-// pass 1 counts the classes, pass 2 ranks every entry by wave ballots plus an
-// exclusive prefix over the waves of each slice.  (Holding a bin's entries
-// in registers for one load phase was slower: the longest lists, which bound
-// the launch, still need two passes.)
+// reference-order list by walk key, into the same range of pairKT (free
+// once the pairs are scattered).  One block of FD_ORDER_THREADS per bin.
+// A lane finds the lanes of its wave with its own key from one ballot per key
+// bit (the AND of the matching ballots), its rank within them by mbcnt; the
+// first such lane adds their number.  Pass 1 totals the keys (and keeps
+// them in LDS for lists of up to FD_ORDER_CACHE entries), pass 2 ranks every
+// entry and offsets it by an exclusive prefix over the waves of each slice.
 __device__ __forceinline__ void tri_order_body(const FdArgs& a, const int bin) {
-    constexpr int NWV = FD_ORDER_THREADS / 64, C = FD_TRI_CLASSES;
-    __shared__ int waveCount[NWV][C];
-    __shared__ int classBase[C];
+    constexpr int NWV = FD_ORDER_THREADS / 64, K = FD_WALK_KEYS;
+    __shared__ int waveCount[NWV][K];
+    __shared__ int keyBase[K];
+    __shared__ uint8_t keyCache[FD_ORDER_CACHE];
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK || s->triOrder != FD_TRI_ORDER_AREA || bin >= s->nb) return;
     const int ts = a.triStart[bin], te = a.triEnd[bin];
     if (te <= ts) return;
     const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int bi = bin & ((1 << s->log2G) - 1), bj = bin >> s->log2G;
     int* out = (int*)a.pairKT;
-    auto cls_of = [&](int i, int& tri) -> int {
+    auto key_of = [&](int i, int& tri) -> int {
         tri = a.sortedTri[i];
-        return ((unsigned)tri < (unsigned)a.ntris) ? (int)a.triClass[tri] : C - 1;
+        if ((unsigned)tri >= (unsigned)a.ntris) return K - 1;
+        return walk_key(a.triAABB[2 * tri], a.triAABB[2 * tri + 1], s, bi, bj);
     };
-    // pass 1: lane c of each wave counts class c
-    int mine = 0;
+    // lanes of the wave with the same key (key K: no entry)
+    auto same_key = [](int key) -> uint64_t {
+        uint64_t m = ~0ull;
+#pragma unroll
+        for (int j = 0; j < FD_WALK_KEY_BITS; ++j) {
+            const uint64_t b = __ballot((key >> j) & 1);
+            m &= ((key >> j) & 1) ? b : ~b;
+        }
+        return m;
+    };
+    for (int k = threadIdx.x; k < K; k += FD_ORDER_THREADS) keyBase[k] = 0;
+    __syncthreads();
+    const bool cached = te - ts <= FD_ORDER_CACHE;
+    // pass 1: key totals
     for (int i0 = ts; i0 < te; i0 += FD_ORDER_THREADS) {
         const int i = i0 + threadIdx.x;
         int tri = 0;
-        const int c = i < te ? cls_of(i, tri) : C;
-#pragma unroll
-        for (int k = 0; k < C; ++k) {
-            const int cnt = __popcll(__ballot(c == k));
-            if (lane == k) mine += cnt;
-        }
+        const int key = i < te ? key_of(i, tri) : K;
+        if (cached && i < te) keyCache[i - ts] = (uint8_t)key;
+        const uint64_t m = same_key(key);
+        if (key < K && mbcnt(m) == 0) atomicAdd(&keyBase[key], __popcll(m));
     }
-    if (lane < C) waveCount[w][lane] = mine;
     __syncthreads();
-    if (threadIdx.x == 0) {   // exclusive prefix over the classes
+    if (threadIdx.x == 0) {   // exclusive prefix over the keys
         int run = 0;
-        for (int k = 0; k < C; ++k) {
-            int tot = 0;
-            for (int j = 0; j < NWV; ++j) tot += waveCount[j][k];
-            classBase[k] = run;
-            run += tot;
+        for (int k = 0; k < K; ++k) {
+            const int x = keyBase[k];
+            keyBase[k] = run;
+            run += x;
         }
     }
     __syncthreads();
     // pass 2: stable scatter, one slice of FD_ORDER_THREADS entries at a time
     for (int i0 = ts; i0 < te; i0 += FD_ORDER_THREADS) {
         const int i = i0 + threadIdx.x;
-        int tri = 0;
-        const int c = i < te ? cls_of(i, tri) : C;
-        int rank = 0;
-#pragma unroll
-        for (int k = 0; k < C; ++k) {
-            const uint64_t m = __ballot(c == k);
-            if (c == k) rank = mbcnt(m);
-            if (lane == k) waveCount[w][k] = __popcll(m);
+        int tri = 0, key = K;
+        if (i < te) {
+            if (cached) {
+                tri = a.sortedTri[i];
+                key = keyCache[i - ts];
+            } else {
+                key = key_of(i, tri);
+            }
         }
+        for (int k = lane; k < K; k += 64) waveCount[w][k] = 0;
+        const uint64_t m = same_key(key);
+        const int rank = mbcnt(m);
+        if (key < K && rank == 0) waveCount[w][key] = __popcll(m);
         __syncthreads();
-        if (threadIdx.x < C) {   // per class: exclusive prefix over the waves, then advance the base
-            int run = classBase[threadIdx.x];
+        for (int k = threadIdx.x; k < K; k += FD_ORDER_THREADS) {   // per key: prefix over the waves
+            int run = keyBase[k];
             for (int j = 0; j < NWV; ++j) {
-                const int x = waveCount[j][threadIdx.x];
-                waveCount[j][threadIdx.x] = run;
+                const int x = waveCount[j][k];
+                waveCount[j][k] = run;
                 run += x;
             }
-            classBase[threadIdx.x] = run;
+            keyBase[k] = run;
         }
         __syncthreads();
-        if (c < C) out[ts + waveCount[w][c] + rank] = tri;
+        if (key < K) out[ts + waveCount[w][key] + rank] = tri;
         __syncthreads();
     }
 }

@@ -144,8 +144,10 @@ int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
  * the order only decides how soon receivers fill up and stop.
  *   FD_TRI_ORDER_LIST: the reference's order (ascending triangle id, the
  *     stable sort of FDc/Step4_RasterizeTriangleToBin.inl:223);
- *   FD_TRI_ORDER_AREA: by area of the triangle's light-plane culling box in
- *     16 powers of two, largest first, ascending id within a class;
+ *   FD_TRI_ORDER_AREA: by how much of the bin's light-plane cell the
+ *     triangle's culling box covers (eighths, whole cell first), then by the
+ *     box's area in 16 powers of two (largest first), ascending id within a
+ *     class;
  *   FD_TRI_ORDER_AUTO (default): decided per frame on the device -- the
  *     list order when consecutive triangles mostly overlap on the light
  *     plane (a connected mesh), else the area order (a triangle soup).

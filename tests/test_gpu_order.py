@@ -1,15 +1,17 @@
 """Walk order of the bins' triangle lists in the occlusion (MI355X only).
 
 fd_set_triangle_order picks the order in which the occlusion kernel walks
-each bin's list: the reference's (ascending triangle id), largest culling
-boxes first (16 power-of-two area classes), or per frame on the device.  A
+each bin's list: the reference's (ascending triangle id), culling boxes that
+cover most of the bin's cell and then the largest first, or per frame on the
+device.  A
 receiver's mask is the OR of its triangles' masks, so the order must not
 change a bit of any output; it only changes how soon receivers fill up.
 
 Checked here: the automatic choice (area order for a triangle soup, list
 order for connected meshes), oracle bits under every order, the area-ordered
-lists as a stable class sort of the reference's lists with classes that obey
-their definition, and the batched form over a soup.
+lists as a stable sort of the reference's lists by walk key (covered share of
+the bin's cell, then area class, restated here in float32) with area classes
+that obey their definition, and the batched form over a soup.
 """
 import numpy as np
 import pytest
@@ -86,7 +88,29 @@ def test_orders_give_identical_bits(fd, wl, which):
     assert np.array_equal(pair_lists[LIST], pair_lists[AREA]) and np.array_equal(pair_lists[LIST], pair_lists[AUTO])
 
 
-def test_area_lists_are_stable_class_sort(fd, wl):
+def _walk_keys(cls, aabb, s, bins_of_pairs, pairs):
+    """Walk key of each pair, restated in float32 with the kernel's
+    expressions: the culling box in bin units ((b - B) * factor), its overlap
+    with the bin's cell, the covered eighths (whole cell first), then the
+    triangle's area class."""
+    f32 = np.float32
+    B0, B1 = f32(s["B"][0]), f32(s["B"][1])
+    f0, f1 = f32(s["factor"][0]), f32(s["factor"][1])
+    log2G = int(s["log2G"])
+    mn, mx = aabb[pairs, 0].astype(f32), aabb[pairs, 1].astype(f32)
+    ux0, ux1 = (mn[:, 0] - B0) * f0, (mx[:, 0] - B0) * f0
+    uy0, uy1 = (mn[:, 1] - B1) * f1, (mx[:, 1] - B1) * f1
+    bi = (bins_of_pairs & ((1 << log2G) - 1)).astype(f32)
+    bj = (bins_of_pairs >> log2G).astype(f32)
+    ox = np.fmax(np.fmin(ux1, bi + f32(1)) - np.fmax(ux0, bi), f32(0))
+    oy = np.fmax(np.fmin(uy1, bj + f32(1)) - np.fmax(uy0, bj), f32(0))
+    cov = (ox * oy).astype(f32)
+    with np.errstate(invalid="ignore"):
+        cc = np.where(cov >= 0, np.minimum((cov * f32(8)).astype(np.int64), 8), 0)
+    return (8 - cc) * 16 + cls[pairs].astype(np.int64)
+
+
+def test_area_lists_are_stable_key_sort(fd, wl):
     frame = _soup(wl)
     print("[order] area lists", flush=True)
     fr = _run(fd, frame, AREA)
@@ -102,7 +126,7 @@ def test_area_lists_are_stable_class_sort(fd, wl):
     fr.close()
     assert len(pairs) == int(s["pairs"]) and len(walk) == len(pairs)
     assert cls.max() < 16
-    # classes against their definition: class k in 1..14 holds boxes of
+    # area classes against their definition: class k in 1..14 holds boxes of
     # 2^(7-k) <= area < 2^(8-k) bin cells, class 0 every area >= 64 cells,
     # class 15 every area < 2^-7 cells (area in float64; boxes within 1e-5 of
     # a class boundary may fall either side under float32 rounding)
@@ -115,16 +139,18 @@ def test_area_lists_are_stable_class_sort(fd, wl):
     ok = (cells >= lo * (1 - 1e-5)) & (cells < hi * (1 + 1e-5))
     assert ok.all(), (int((~ok).sum()), cells[~ok][:5], c[~ok][:5])
     assert len(np.unique(c)) >= 4   # the soup spans several classes
-    # each bin's walked list: the reference's list stably sorted by class
-    nonempty = 0
+    # each bin's walked list: the reference's list stably sorted by walk key
+    nonempty, keysSeen = 0, set()
     for b in range(len(ts)):
         if te[b] <= ts[b]:
             continue
         nonempty += 1
         ref = pairs[ts[b]:te[b]]
-        exp = ref[np.argsort(cls[ref], kind="stable")]
+        keys = _walk_keys(cls, aabb, s, np.full(len(ref), b, np.int64), ref)
+        keysSeen.update(np.unique(keys // 16).tolist())
+        exp = ref[np.argsort(keys, kind="stable")]
         assert np.array_equal(walk[ts[b]:te[b]], exp), b
-    assert nonempty > 0
+    assert nonempty > 0 and len(keysSeen) >= 3   # several coverage classes occur
 
 
 def test_batch_over_soup(fd, wl):

@@ -202,7 +202,7 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     const long np = std::max(std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)),
                              fdk::scan_partials_needed(T)) + 1;
     a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));
-    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), 3 * a.nblkT) * sizeof(uint32_t));
+    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), a.nblkT) * sizeof(uint32_t));
     a.opCount = countOps ? (unsigned long long*)take(FD_NOPS * sizeof(unsigned long long)) : nullptr;
     if (!a.masks) a.masks = (uint32_t*)take(WH * a.L.res * a.L.wpr * sizeof(uint32_t));
     return off;

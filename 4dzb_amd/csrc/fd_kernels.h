@@ -7,9 +7,7 @@
 #define FD_RXB 1024   // receivers per block in the key / scatter passes
 #define FD_TRB 1024   // triangles per block in the count / scatter passes
 #define FD_TRI_CLASSES 16      // area classes of the area walk order
-#define FD_COVER_CLASSES 9     // bin-cell coverage classes of the area walk order (eighths)
-#define FD_WALK_KEYS (FD_COVER_CLASSES * FD_TRI_CLASSES)
-#define FD_WALK_KEY_BITS 8     // bits of a walk key or of the no-entry value FD_WALK_KEYS
+#define FD_WALK_KEY_BITS 5     // bits of an area class or of the no-entry value FD_TRI_CLASSES
 #define FD_ORDER_CACHE 32768   // list entries whose walk keys the area-order pass keeps in LDS
 #define FD_ORDER_THREADS 1024  // threads per bin in the area-order pass
 
@@ -119,7 +117,7 @@ enum FdMap { FD_MAP_ONE, FD_MAP_BINS, FD_MAP_RED, FD_MAP_RX, FD_MAP_SCANR, FD_MA
              FD_MAP_OCC1, FD_MAP_OCC2, FD_MAP_OCC3, FD_MAP_OCC4, FD_MAP_OCC5, FD_MAP_ZERO, FD_MAP_FIN, FD_MAP_SCANO,
              FD_MAP_NB, FD_NMAPS };
 static_assert(FD_MAP_OCC0 + FD_OCC_CLASSES == FD_MAP_ZERO, "one block map per occlusion class");
-static_assert(FD_WALK_KEYS < (1 << FD_WALK_KEY_BITS), "walk keys and the no-entry value fit the key bits");
+static_assert(FD_TRI_CLASSES < (1 << FD_WALK_KEY_BITS), "area classes and the no-entry value fit the key bits");
 
 struct FdBatchPlan {
     const FdArgs* inst;       // device: instance table (n)

@@ -644,12 +644,15 @@ __device__ __forceinline__ void walk_rect_wave(int sx, int sy, int ex, int ey, F
     }
 }
 
-// Area class of a triangle's light-plane culling box: 7 - floor(log2(area in
-// bin cells)) clamped to [0, FD_TRI_CLASSES - 1]; 0 holds the largest boxes
-// (64 cells and up), the last class boxes under 1/256 of a cell and the
-// degenerate ones (area zero, denormal or NaN).
-__device__ __forceinline__ int area_class(const float4 mn, const float4 mx, const FdState* s) {
-    const float cells = ((mx.x - mn.x) * s->factor[0]) * ((mx.y - mn.y) * s->factor[1]);
+// Area class of a triangle projected on the light plane (its three
+// makeRectangle points): 7 - floor(log2(area in bin cells)) clamped to
+// [0, FD_TRI_CLASSES - 1]; 0 holds the largest triangles (64 cells and up),
+// the last class those under 1/256 of a cell and the degenerate ones (area
+// zero, denormal or NaN).  The area is
+// |(x1 - x0)(y2 - y0) - (y1 - y0)(x2 - x0)| / 2 in float, in that order.
+__device__ __forceinline__ int area_class(const float (&px)[3], const float (&py)[3], const FdState* s) {
+    const float twice = fabsf((px[1] - px[0]) * (py[2] - py[0]) - (py[1] - py[0]) * (px[2] - px[0]));
+    const float cells = ((twice * 0.5f) * s->factor[0]) * s->factor[1];
     if (!(cells > 0.0f)) return FD_TRI_CLASSES - 1;
     const int e = (int)(__float_as_uint(cells) >> 23) - 127;   // floor(log2), -127 for denormals
     return min(max(7 - e, 0), FD_TRI_CLASSES - 1);
@@ -657,7 +660,7 @@ __device__ __forceinline__ int area_class(const float4 mn, const float4 mx, cons
 
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t thist[];  // nb counters, then the staged bin statistics
-    __shared__ uint32_t blkValid, blkPairs, blkOverlaps;
+    __shared__ uint32_t blkValid;
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
@@ -665,7 +668,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     float4* binQ = (float4*)(thist + nb);
     float* binW = (float*)(binQ + nb);
     for (int k = threadIdx.x; k < nb; k += blockDim.x) thist[k] = 0;
-    if (threadIdx.x == 0) blkValid = blkPairs = blkOverlaps = 0;
+    if (threadIdx.x == 0) blkValid = 0;
     if (staged) stage_bins(a, nb, binQ, binW);
     __syncthreads();
     const int t = blk * FD_TRB + threadIdx.x;
@@ -700,12 +703,15 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
                 for (int j = 0; j < 3; ++j)
                     culled = culled && (dot3(nx, ny, nz, c[k][0] - vv[j][0], c[k][1] - vv[j][1], c[k][2] - vv[j][2]) <= 0);
         }
+        float px[3] = {0.0f, 0.0f, 0.0f}, py[3] = {0.0f, 0.0f, 0.0f};
         if (!culled) {
             const float* vv[3] = {v0, v1, v2};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {  // getTriAABB (:37-54)
                 float rx, ry, rz;
                 makeRect(m, vv[k][0], vv[k][1], vv[k][2], rx, ry, rz);
+                px[k] = rx;
+                py[k] = ry;
                 mn.x = fmin_(mn.x, rx);
                 mn.y = fmin_(mn.y, ry);
                 mn.z = fmin_(mn.z, rz);
@@ -735,23 +741,11 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         }
         a.triAABB[2 * t] = mn;
         a.triAABB[2 * t + 1] = mx;
-        a.triClass[t] = (uint8_t)(valid ? area_class(mn, mx, s) : FD_TRI_CLASSES - 1);
+        a.triClass[t] = (uint8_t)(valid ? area_class(px, py, s) : FD_TRI_CLASSES - 1);
     }
     {
         const int nv = wave_sum_i32(valid);
         if (lane_id() == 0 && nv) atomicAdd(&blkValid, (uint32_t)nv);
-        // list coherence: how often a valid triangle's culling box overlaps
-        // the next triangle's (the next lane's; the wave's last lane has none)
-        const float nx0 = __shfl_down(mn.x, 1, 64), ny0 = __shfl_down(mn.y, 1, 64);
-        const float nx1 = __shfl_down(mx.x, 1, 64), ny1 = __shfl_down(mx.y, 1, 64);
-        const int nvld = __shfl_down(valid, 1, 64);
-        const bool pair = valid && nvld && lane_id() < 63;
-        const bool over = pair && mn.x <= nx1 && nx0 <= mx.x && mn.y <= ny1 && ny0 <= mx.y;
-        const int np = __popcll(__ballot(pair)), no = __popcll(__ballot(over));
-        if (lane_id() == 0 && np) {
-            atomicAdd(&blkPairs, (uint32_t)np);
-            atomicAdd(&blkOverlaps, (uint32_t)no);
-        }
     }
     // count pass
     TriClamp c = tri_clamp(mn, mx, s);
@@ -802,11 +796,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
     }
     __syncthreads();
-    if (threadIdx.x == 0) {   // totalled in k_post_pairs
-        a.part[blk] = blkValid;
-        a.part[a.nblkT + blk] = blkPairs;
-        a.part[2 * a.nblkT + blk] = blkOverlaps;
-    }
+    if (threadIdx.x == 0) a.part[blk] = blkValid;   // totalled in k_post_pairs
     for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blk] = thist[k];
 }
 
@@ -1081,33 +1071,17 @@ __device__ __forceinline__ void tables_body(const FdArgs& a, const int blk) {
     }
 }
 
-// Walk key of a (bin, triangle) pair for the area order: first how much of
-// the bin's light-plane cell the triangle's culling box covers (eighths,
-// rounded down; a box covering the whole cell first), then the box's area
-// class.  Both are read from the culling box in the float expressions the
-// tests restate: the box in bin units is (b - B) * factor, as in the pair
-// count's clamp.
-__device__ __forceinline__ int walk_key(const float4 mn, const float4 mx, const FdState* s, int bi, int bj) {
-    const float ux0 = (mn.x - s->B[0]) * s->factor[0], ux1 = (mx.x - s->B[0]) * s->factor[0];
-    const float uy0 = (mn.y - s->B[1]) * s->factor[1], uy1 = (mx.y - s->B[1]) * s->factor[1];
-    const float ox = fmaxf(fminf(ux1, (float)(bi + 1)) - fmaxf(ux0, (float)bi), 0.0f);
-    const float oy = fmaxf(fminf(uy1, (float)(bj + 1)) - fmaxf(uy0, (float)bj), 0.0f);
-    const float cov = ox * oy;   // NaN for a NaN box: class 0 below, then the last coverage class
-    const int cc = cov >= 0.0f ? min((int)(cov * 8.0f), FD_COVER_CLASSES - 1) : 0;
-    return (FD_COVER_CLASSES - 1 - cc) * FD_TRI_CLASSES + area_class(mn, mx, s);
-}
-
 // Area order of one bin's triangle list (frames whose walk order is
 // FD_TRI_ORDER_AREA): a stable counting sort of triStart..triEnd of the
-// reference-order list by walk key, into the same range of pairKT (free
-// once the pairs are scattered).  One block of FD_ORDER_THREADS per bin.
-// A lane finds the lanes of its wave with its own key from one ballot per key
+// reference-order list by the triangles' area classes, into the same range
+// of pairKT (free once the pairs are scattered).  One block of
+// FD_ORDER_THREADS per bin.  A lane finds the lanes of its wave with its own key from one ballot per key
 // bit (the AND of the matching ballots), its rank within them by mbcnt; the
 // first such lane adds their number.  Pass 1 totals the keys (and keeps
 // them in LDS for lists of up to FD_ORDER_CACHE entries), pass 2 ranks every
 // entry and offsets it by an exclusive prefix over the waves of each slice.
 __device__ __forceinline__ void tri_order_body(const FdArgs& a, const int bin) {
-    constexpr int NWV = FD_ORDER_THREADS / 64, K = FD_WALK_KEYS;
+    constexpr int NWV = FD_ORDER_THREADS / 64, K = FD_TRI_CLASSES;
     __shared__ int waveCount[NWV][K];
     __shared__ int keyBase[K];
     __shared__ uint8_t keyCache[FD_ORDER_CACHE];
@@ -1116,12 +1090,10 @@ __device__ __forceinline__ void tri_order_body(const FdArgs& a, const int bin) {
     const int ts = a.triStart[bin], te = a.triEnd[bin];
     if (te <= ts) return;
     const int lane = lane_id(), w = threadIdx.x >> 6;
-    const int bi = bin & ((1 << s->log2G) - 1), bj = bin >> s->log2G;
     int* out = (int*)a.pairKT;
     auto key_of = [&](int i, int& tri) -> int {
         tri = a.sortedTri[i];
-        if ((unsigned)tri >= (unsigned)a.ntris) return K - 1;
-        return walk_key(a.triAABB[2 * tri], a.triAABB[2 * tri + 1], s, bi, bj);
+        return ((unsigned)tri < (unsigned)a.ntris) ? (int)a.triClass[tri] : K - 1;
     };
     // lanes of the wave with the same key (key K: no entry)
     auto same_key = [](int key) -> uint64_t {
@@ -1196,38 +1168,20 @@ __device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) 
     if (s->status != FD_EXIT_OK) return;
     // valid triangles: the pair-count blocks' totals (no count pass without
     // triangles)
-    __shared__ int wsum[3][FD_FIN_THREADS / 64];
-    int v = 0, np = 0, no = 0;
+    __shared__ int wsum[FD_FIN_THREADS / 64];
+    int v = 0;
     if (a.ntris > 0)
-        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) {
-            v += (int)a.part[i];
-            np += (int)a.part[a.nblkT + i];
-            no += (int)a.part[2 * a.nblkT + i];
-        }
+        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) v += (int)a.part[i];
     v = wave_sum_i32(v);
-    np = wave_sum_i32(np);
-    no = wave_sum_i32(no);
-    if ((threadIdx.x & 63) == 0) {
-        wsum[0][threadIdx.x >> 6] = v;
-        wsum[1][threadIdx.x >> 6] = np;
-        wsum[2][threadIdx.x >> 6] = no;
-    }
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x != 0) return;
     int nv = 0;
-    long pairsAdj = 0, overlaps = 0;
-    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) {
-        nv += wsum[0][j];
-        pairsAdj += wsum[1][j];
-        overlaps += wsum[2][j];
-    }
+    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) nv += wsum[j];
     s->validTris = nv;
-    // walk order of the bins' lists: the reference's (ascending id) when
-    // neighbouring triangles mostly overlap (a connected mesh: the list order
-    // already sweeps the surface), else largest culling boxes first (a
-    // triangle soup: big occluders fill receivers early)
-    s->triOrder = a.triOrder != FD_TRI_ORDER_AUTO ? a.triOrder
-                  : (2 * overlaps >= pairsAdj ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_AREA);
+    // walk order of the bins' lists: largest projected triangles first
+    // unless the caller asked for the reference's order
+    s->triOrder = a.triOrder == FD_TRI_ORDER_LIST ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_AREA;
     // valid receivers: binned (keys < nb, the scanned histogram at key nb)
     // plus keys out of range
     s->validSamples = (int)a.histR[(size_t)s->nb * a.nblkR] + s->keyOutOfRange;

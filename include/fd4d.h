@@ -144,13 +144,11 @@ int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
  * the order only decides how soon receivers fill up and stop.
  *   FD_TRI_ORDER_LIST: the reference's order (ascending triangle id, the
  *     stable sort of FDc/Step4_RasterizeTriangleToBin.inl:223);
- *   FD_TRI_ORDER_AREA: by how much of the bin's light-plane cell the
- *     triangle's culling box covers (eighths, whole cell first), then by the
- *     box's area in 16 powers of two (largest first), ascending id within a
+ *   FD_TRI_ORDER_AREA: by the area of the triangle projected on the light
+ *     plane in 16 powers of two, largest first, ascending id within a
  *     class;
- *   FD_TRI_ORDER_AUTO (default): decided per frame on the device -- the
- *     list order when consecutive triangles mostly overlap on the light
- *     plane (a connected mesh), else the area order (a triangle soup).
+ *   FD_TRI_ORDER_AUTO (default): the area order (measured faster for
+ *     triangle soups and connected meshes alike).
  * The batched and serving forms use the context's / the default setting;
  * fd_debug_read("pair_tri") always returns the reference's order. */
 #define FD_TRI_ORDER_AUTO 0

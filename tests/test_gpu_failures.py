@@ -159,9 +159,10 @@ def test_unsupported_light_grid(fd, oracle, wl):
 def test_counting_kernel(fd, oracle, wl, light_res):
     """The profiling frame (fd_set_profiling count_ops) runs a counting
     variant of the occlusion kernel with one wave per work item.  Its masks
-    equal the timed kernel's and the oracle's, and its counters equal the
-    oracle's loop counts: candidate tests, cull passes, planes set up and
-    rows visited by the reference."""
+    equal the timed kernel's and the oracle's, and in the reference's walk
+    order (ascending triangle id) its counters equal the oracle's loop
+    counts: candidate tests, cull passes, planes set up and rows visited by
+    the reference.  In the default (area) order the masks stay the same."""
     frame = wl.heightfield_1k(res=384, nsph=80, light_res=light_res)
     o = oracle.run(frame)
     assert o.exit == 0
@@ -170,6 +171,12 @@ def test_counting_kernel(fd, oracle, wl, light_res):
     assert rc == 0
     sh0, mk0 = (x.copy() for x in run.outputs())
     run.ctx.set_profiling(stage_events=True, count_ops=True)
+    rc, _ = run.run()
+    assert rc == 0
+    sh, mk = run.outputs()
+    assert np.array_equal(sh.view(np.uint32), sh0.view(np.uint32))
+    assert np.array_equal(mk, mk0)
+    run.ctx.set_triangle_order(fd.TRI_ORDER_LIST)
     rc, st = run.run()
     assert rc == 0
     assert st.work_items > 2048   # unsplit triangle lists: each receiver meets its whole list
@@ -185,6 +192,7 @@ def test_counting_kernel(fd, oracle, wl, light_res):
     assert got == want, (got, want)
     assert ops[2] <= ops[6]   # rows the kernel visits never exceed the reference's
     run.ctx.set_profiling(stage_events=False, count_ops=False)
+    run.ctx.set_triangle_order(fd.TRI_ORDER_AUTO)
     rc, _ = run.run()
     assert rc == 0
     run.close()

@@ -1,17 +1,15 @@
 """Walk order of the bins' triangle lists in the occlusion (MI355X only).
 
 fd_set_triangle_order picks the order in which the occlusion kernel walks
-each bin's list: the reference's (ascending triangle id), culling boxes that
-cover most of the bin's cell and then the largest first, or per frame on the
-device.  A
-receiver's mask is the OR of its triangles' masks, so the order must not
+each bin's list: the reference's (ascending triangle id) or the largest
+triangles on the light plane first (16 power-of-two area classes, the
+default).  A receiver's mask is the OR of its triangles' masks, so the order must not
 change a bit of any output; it only changes how soon receivers fill up.
 
-Checked here: the automatic choice (area order for a triangle soup, list
-order for connected meshes), oracle bits under every order, the area-ordered
-lists as a stable sort of the reference's lists by walk key (covered share of
-the bin's cell, then area class, restated here in float32) with area classes
-that obey their definition, and the batched form over a soup.
+Checked here: the default choice, oracle bits under every order, the
+area-ordered lists as a stable sort of the reference's lists by the
+triangles' area classes (restated here in float32, and checked against their
+definition), and the batched form over a soup.
 """
 import numpy as np
 import pytest
@@ -55,7 +53,7 @@ def _soup_small(wl):
     return wl.boxes_1080p(w=160, h=90, ntri=60_000, light_res=40)
 
 
-@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", LIST)])
+@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", AREA)])
 def test_auto_choice_and_oracle(fd, wl, oracle, which, expect):
     frame = {"soup": lambda: _soup(wl), "soup32": lambda: _soup(wl, 32), "soup-small": lambda: _soup_small(wl),
              "mesh": lambda: _mesh(wl)}[which]()
@@ -88,29 +86,31 @@ def test_orders_give_identical_bits(fd, wl, which):
     assert np.array_equal(pair_lists[LIST], pair_lists[AREA]) and np.array_equal(pair_lists[LIST], pair_lists[AUTO])
 
 
-def _walk_keys(cls, aabb, s, bins_of_pairs, pairs):
-    """Walk key of each pair, restated in float32 with the kernel's
-    expressions: the culling box in bin units ((b - B) * factor), its overlap
-    with the bin's cell, the covered eighths (whole cell first), then the
-    triangle's area class."""
+def _area_classes(tris, light, s):
+    """Area class of each triangle record, restated in float32 with the
+    kernel's expressions: the three makeRectangle points, twice the area
+    |(x1 - x0)(y2 - y0) - (y1 - y0)(x2 - x0)|, halved and scaled to bin
+    cells, 7 - floor(log2) from the exponent bits, clamped to 0..15."""
     f32 = np.float32
-    B0, B1 = f32(s["B"][0]), f32(s["B"][1])
-    f0, f1 = f32(s["factor"][0]), f32(s["factor"][1])
-    log2G = int(s["log2G"])
-    mn, mx = aabb[pairs, 0].astype(f32), aabb[pairs, 1].astype(f32)
-    ux0, ux1 = (mn[:, 0] - B0) * f0, (mx[:, 0] - B0) * f0
-    uy0, uy1 = (mn[:, 1] - B1) * f1, (mx[:, 1] - B1) * f1
-    bi = (bins_of_pairs & ((1 << log2G) - 1)).astype(f32)
-    bj = (bins_of_pairs >> log2G).astype(f32)
-    ox = np.fmax(np.fmin(ux1, bi + f32(1)) - np.fmax(ux0, bi), f32(0))
-    oy = np.fmax(np.fmin(uy1, bj + f32(1)) - np.fmax(uy0, bj), f32(0))
-    cov = (ox * oy).astype(f32)
-    with np.errstate(invalid="ignore"):
-        cc = np.where(cov >= 0, np.minimum((cov * f32(8)).astype(np.int64), 8), 0)
-    return (8 - cc) * 16 + cls[pairs].astype(np.int64)
+    m = light["m"].astype(f32)
+
+    def rect(v):
+        x, y, z = v[:, 0], v[:, 1], v[:, 2]
+        rz = ((m[3] * x + m[7] * y) + m[11] * z) + m[15] * f32(1)
+        rx = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * f32(1)
+        ry = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * f32(1)
+        return rx / rz * f32(0.5) + f32(0.5), ry / rz * f32(0.5) + f32(0.5)
+
+    with np.errstate(all="ignore"):
+        p = [rect(tris[k].astype(f32)) for k in ("v0", "v1", "v2")]
+        twice = np.abs((p[1][0] - p[0][0]) * (p[2][1] - p[0][1]) - (p[1][1] - p[0][1]) * (p[2][0] - p[0][0]))
+        cells = ((twice * f32(0.5)) * f32(s["factor"][0])) * f32(s["factor"][1])
+        e = (cells.astype(f32).view(np.uint32) >> 23).astype(np.int64) - 127
+        cls = np.clip(7 - e, 0, 15)
+        return np.where(cells > 0, cls, 15), cells.astype(np.float64)
 
 
-def test_area_lists_are_stable_key_sort(fd, wl):
+def test_area_lists_are_stable_class_sort(fd, wl):
     frame = _soup(wl)
     print("[order] area lists", flush=True)
     fr = _run(fd, frame, AREA)
@@ -118,7 +118,8 @@ def test_area_lists_are_stable_key_sort(fd, wl):
     s = fd.read_state(ctx)
     assert int(s["triOrder"]) == AREA
     cls = ctx.debug_read("tri_class", np.uint8)
-    aabb = ctx.debug_read("tri_aabb", np.float32).reshape(-1, 2, 4)
+    tris = fd.read_tris(ctx)
+    light = fd.read_light(ctx)
     pairs = ctx.debug_read("pair_tri", np.int32)
     walk = ctx.debug_read("pair_tri_walk", np.int32)
     ts = ctx.debug_read("bin_tri_start", np.int32)
@@ -126,31 +127,30 @@ def test_area_lists_are_stable_key_sort(fd, wl):
     fr.close()
     assert len(pairs) == int(s["pairs"]) and len(walk) == len(pairs)
     assert cls.max() < 16
-    # area classes against their definition: class k in 1..14 holds boxes of
-    # 2^(7-k) <= area < 2^(8-k) bin cells, class 0 every area >= 64 cells,
-    # class 15 every area < 2^-7 cells (area in float64; boxes within 1e-5 of
-    # a class boundary may fall either side under float32 rounding)
     used = np.unique(pairs)
-    mn, mx = aabb[used, 0].astype(np.float64), aabb[used, 1].astype(np.float64)
-    cells = (mx[:, 0] - mn[:, 0]) * float(s["factor"][0]) * (mx[:, 1] - mn[:, 1]) * float(s["factor"][1])
-    c = cls[used].astype(np.int64)
+    # the device's classes equal the float32 restatement ...
+    exp, cells = _area_classes(tris[used], light, s)
+    assert np.array_equal(cls[used].astype(np.int64), exp)
+    # ... and obey their definition: class k in 1..14 holds triangles of
+    # 2^(7-k) <= area < 2^(8-k) bin cells, class 0 every area >= 64 cells,
+    # class 15 every area < 2^-7 cells (area from the float32 points; within
+    # 1e-5 of a class boundary either side)
+    c = exp
     lo = np.where(c == 15, 0.0, 2.0 ** (7 - c))
     hi = np.where(c == 0, np.inf, 2.0 ** (8 - c))
     ok = (cells >= lo * (1 - 1e-5)) & (cells < hi * (1 + 1e-5))
     assert ok.all(), (int((~ok).sum()), cells[~ok][:5], c[~ok][:5])
     assert len(np.unique(c)) >= 4   # the soup spans several classes
-    # each bin's walked list: the reference's list stably sorted by walk key
-    nonempty, keysSeen = 0, set()
+    # each bin's walked list: the reference's list stably sorted by class
+    nonempty = 0
     for b in range(len(ts)):
         if te[b] <= ts[b]:
             continue
         nonempty += 1
         ref = pairs[ts[b]:te[b]]
-        keys = _walk_keys(cls, aabb, s, np.full(len(ref), b, np.int64), ref)
-        keysSeen.update(np.unique(keys // 16).tolist())
-        exp = ref[np.argsort(keys, kind="stable")]
-        assert np.array_equal(walk[ts[b]:te[b]], exp), b
-    assert nonempty > 0 and len(keysSeen) >= 3   # several coverage classes occur
+        expw = ref[np.argsort(cls[ref], kind="stable")]
+        assert np.array_equal(walk[ts[b]:te[b]], expw), b
+    assert nonempty > 0
 
 
 def test_batch_over_soup(fd, wl):

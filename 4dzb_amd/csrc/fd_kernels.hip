@@ -646,9 +646,9 @@ __device__ __forceinline__ void walk_rect_wave(int sx, int sy, int ex, int ey, F
 
 // Area class of a triangle projected on the light plane (its three
 // makeRectangle points): 7 - floor(log2(area in bin cells)) clamped to
-// [0, FD_TRI_CLASSES - 1]; 0 holds the largest triangles (64 cells and up),
-// the last class those under 1/256 of a cell and the degenerate ones (area
-// zero, denormal or NaN).  The area is
+// [0, FD_TRI_CLASSES - 1]; 0 holds the largest triangles (128 cells and up,
+// infinite areas too), the last class those under 1/128 of a cell and the
+// degenerate ones (area zero or NaN).  The area is
 // |(x1 - x0)(y2 - y0) - (y1 - y0)(x2 - x0)| / 2 in float, in that order.
 __device__ __forceinline__ int area_class(const float (&px)[3], const float (&py)[3], const FdState* s) {
     const float twice = fabsf((px[1] - px[0]) * (py[2] - py[0]) - (py[1] - py[0]) * (px[2] - px[0]));

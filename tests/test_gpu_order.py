@@ -132,7 +132,7 @@ def test_area_lists_are_stable_class_sort(fd, wl):
     exp, cells = _area_classes(tris[used], light, s)
     assert np.array_equal(cls[used].astype(np.int64), exp)
     # ... and obey their definition: class k in 1..14 holds triangles of
-    # 2^(7-k) <= area < 2^(8-k) bin cells, class 0 every area >= 64 cells,
+    # 2^(7-k) <= area < 2^(8-k) bin cells, class 0 every area >= 128 cells,
     # class 15 every area < 2^-7 cells (area from the float32 points; within
     # 1e-5 of a class boundary either side)
     c = exp

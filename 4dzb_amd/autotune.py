@@ -5,7 +5,14 @@ representative instance under every candidate layout (bins per light-plane
 side x receiver tile width x waves per occlusion work item) through the
 library's fd_autotune, which times each on the device and compares every
 output bit with the default layout (16x16 bins, 8x8 tiles, the library's
-default waves).  The fastest bit-exact layout wins.  The bin grid
+default waves).  The fastest bit-exact layout wins.  Optionally (orders=True)
+the two walk orders of the bins' triangle lists -- the reference's, and
+largest projected triangles first -- are then timed at that layout on up to
+ORDER_SAMPLES instances of the class, and the area order is kept when it is at
+least 3% faster over them (the automatic choice per frame keeps the
+reference's order for connected meshes, where the area order wins or loses
+by up to 40% depending on the light pose, so one instance is not enough to
+decide a class).  The bin grid
 also sets the receiver sort's key width (one counting-sort pass over
 2*log2(bins) + 1 bits), so picking the grid picks the sort's radix width.
 
@@ -23,6 +30,8 @@ BINS = (8, 16, 32)
 TILES = (8, 16, 32)
 WAVES = (4, 8)
 CANDIDATES = [(b, t, w) for w in WAVES for b in BINS for t in TILES]
+ORDER_LIST, ORDER_AREA = 1, 2   # FD_TRI_ORDER_LIST / FD_TRI_ORDER_AREA
+ORDER_SAMPLES = 8               # instances of a class that decide its walk order
 
 
 def default_waves(res):
@@ -39,31 +48,50 @@ def class_key(scene, w, h, res):
     return f"{scene}:{w}x{h}:{res}"
 
 
-def tune_one(ctx, item, candidates=CANDIDATES, reps=3):
+def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None):
     """Tune one instance class on `ctx`'s scene.  item: a Batch item (pos,
-    nrm, w, h, light, shadow).  Returns the cache entry."""
+    nrm, w, h, light, shadow).  order_items: instances of the class on which
+    to time the two walk orders at the chosen layout (None: no order
+    choice, the automatic one).  Returns the cache entry."""
     t0 = time.perf_counter()
     best, ms = ctx.autotune(item, candidates, reps)
     dl = default_layout(item["light"].res_w)
     default = ms[candidates.index(dl)] if dl in candidates else None
-    b, t, w = candidates[best] if best >= 0 else dl   # nothing bit-exact besides the reference: the default
-    return dict(bins=b, tile=t, waves=w, ms=(ms[best] if best >= 0 else default), default_ms=default,
-                candidates={f"{cb}x{ct}x{cw}": (round(m, 4) if m >= 0 else ("not exact" if m == -1 else "pair arena too small"))
-                            for (cb, ct, cw), m in zip(candidates, ms)},
+    b, t, w = candidates[best][:3] if best >= 0 else dl   # nothing bit-exact besides the reference: the default
+    best_ms = ms[best] if best >= 0 else default
+    # the walk order at that layout, summed over the given instances
+    order, order_ms = 0, {}
+    if order_items:
+        tl = ta = 0.0
+        for it in order_items:
+            _, oms = ctx.autotune(it, [(b, t, w, ORDER_LIST), (b, t, w, ORDER_AREA)], reps)
+            if min(oms) < 0:   # not exact or no fit (cannot happen: the order changes no bit)
+                tl = ta = -1.0
+                break
+            tl, ta = tl + oms[0], ta + oms[1]
+        order_ms = {"list": round(tl, 4), "area": round(ta, 4), "instances": len(order_items)}
+        if tl > 0 and ta > 0:
+            order = ORDER_AREA if ta < 0.97 * tl else ORDER_LIST
+    return dict(bins=b, tile=t, waves=w, order=order, ms=best_ms, default_ms=default,
+                candidates={"x".join(str(x) for x in c[:3]): (round(m, 4) if m >= 0 else ("not exact" if m == -1 else "pair arena too small"))
+                            for c, m in zip(candidates, ms)},
+                order_ms=order_ms,
                 tune_s=round(time.perf_counter() - t0, 3))
 
 
-def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3):
+def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3, orders=False):
     """items: Batch items; one representative per class (scene, w, h, res) is
-    tuned unless the cache already holds it.  Returns the cache (dict)."""
+    tuned unless the cache already holds it (with orders, up to
+    ORDER_SAMPLES of the class's items decide its walk order).  Returns the
+    cache (dict)."""
     cache = {} if cache is None else cache
-    seen = set()
+    groups = {}
     for it in items:
-        key = class_key(scene, it["w"], it["h"], it["light"].res_w)
-        if key in cache or key in seen:
+        groups.setdefault(class_key(scene, it["w"], it["h"], it["light"].res_w), []).append(it)
+    for key, its in groups.items():
+        if key in cache:
             continue
-        seen.add(key)
-        cache[key] = tune_one(ctx, it, candidates, reps)
+        cache[key] = tune_one(ctx, its[0], candidates, reps, order_items=its[:ORDER_SAMPLES] if orders else None)
     return cache
 
 
@@ -73,6 +101,12 @@ def layout(cache, scene, w, h, res):
     return (e["bins"], e["tile"], e.get("waves", default_waves(res))) if e else default_layout(res)
 
 
+def order(cache, scene, w, h, res):
+    """Walk order of a class (ORDER_LIST / ORDER_AREA), 0 = automatic."""
+    e = cache.get(class_key(scene, w, h, res))
+    return e.get("order", 0) if e else 0
+
+
 def server_table(cache):
     """The cache as the server's per-class table (fd_server_config.tune)."""
     out = []
@@ -80,7 +114,7 @@ def server_table(cache):
         scene, wh, res = key.split(":")
         w, h = wh.split("x")
         out.append(dict(scene=int(scene), w=int(w), h=int(h), res=int(res), bins=e["bins"], tile=e["tile"],
-                        waves=e.get("waves", default_waves(int(res))), ms=e["ms"] or 0.0))
+                        waves=e.get("waves", default_waves(int(res))), order=e.get("order", 0), ms=e["ms"] or 0.0))
     return out
 
 

@@ -202,7 +202,7 @@ size_t layout(FdArgs& a, char* base, bool countOps) {
     const long np = std::max(std::max(fdk::scan_partials_needed(a.nR), fdk::scan_partials_needed(a.nT)),
                              fdk::scan_partials_needed(T)) + 1;
     a.scanPart = (uint64_t*)take(np * sizeof(uint64_t));
-    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), a.nblkT) * sizeof(uint32_t));
+    a.part = (uint32_t*)take((size_t)std::max(8 * (a.nbR + a.nbV), 3 * a.nblkT) * sizeof(uint32_t));
     a.opCount = countOps ? (unsigned long long*)take(FD_NOPS * sizeof(unsigned long long)) : nullptr;
     if (!a.masks) a.masks = (uint32_t*)take(WH * a.L.res * a.L.wpr * sizeof(uint32_t));
     return off;
@@ -796,10 +796,14 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
         FdLight L;
         light_setup(q.light, L);
         if (q.occ_waves != 0 && !fd_valid_waves(q.occ_waves)) { delete b; return FD_ERR_ARG; }
+        if (q.tri_order != FD_TRI_ORDER_AUTO && q.tri_order != FD_TRI_ORDER_LIST && q.tri_order != FD_TRI_ORDER_AREA) {
+            delete b;
+            return FD_ERR_ARG;
+        }
         const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
                   c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
-        b->args[i].triOrder = c->triOrder;
+        b->args[i].triOrder = q.tri_order ? q.tri_order : c->triOrder;
         b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves ? c->occWaves : fd_default_waves(L.wpr);
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
@@ -889,11 +893,12 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
     // one layout: returns a status, the median device time and the outputs
-    auto run = [&](int bins, int tw, int waves, uint32_t* dst, float* ms) -> int {
+    auto run = [&](int bins, int tw, int waves, int order, uint32_t* dst, float* ms) -> int {
         fd_instance q = *inst;
         q.bins_per_side = bins;
         q.tile_width = tw;
         q.occ_waves = waves;
+        q.tri_order = order;
         q.d_shadow = (float*)dst;
         q.d_masks = dst + WH;
         fd_batch* b = nullptr;
@@ -921,11 +926,12 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     };
     int rc = hip_status(e);
     float refMs = 0.0f;
-    if (rc == FD_OK) rc = run(16, 8, 0, (uint32_t*)ref.p, &refMs);
+    if (rc == FD_OK) rc = run(16, 8, 0, 0, (uint32_t*)ref.p, &refMs);
     *best = -1;
     for (int i = 0; i < n && rc == FD_OK; ++i) {
         float ms = 0.0f;
-        rc = run(cands[i].bins_per_side, cands[i].tile_width, cands[i].occ_waves, (uint32_t*)out.p, &ms);
+        rc = run(cands[i].bins_per_side, cands[i].tile_width, cands[i].occ_waves, cands[i].tri_order, (uint32_t*)out.p,
+                 &ms);
         if (rc == FD_ERR_CAPACITY) {   // this grid needs more pairs than the instance's arena
             cands[i].ms = -2.0f;
             rc = FD_OK;

@@ -660,7 +660,7 @@ __device__ __forceinline__ int area_class(const float (&px)[3], const float (&py
 
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     extern __shared__ uint32_t thist[];  // nb counters, then the staged bin statistics
-    __shared__ uint32_t blkValid;
+    __shared__ uint32_t blkValid, blkPairs, blkOverlaps;
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
     const int nb = s->nb, log2G = s->log2G;
@@ -668,7 +668,7 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     float4* binQ = (float4*)(thist + nb);
     float* binW = (float*)(binQ + nb);
     for (int k = threadIdx.x; k < nb; k += blockDim.x) thist[k] = 0;
-    if (threadIdx.x == 0) blkValid = 0;
+    if (threadIdx.x == 0) blkValid = blkPairs = blkOverlaps = 0;
     if (staged) stage_bins(a, nb, binQ, binW);
     __syncthreads();
     const int t = blk * FD_TRB + threadIdx.x;
@@ -746,6 +746,18 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     {
         const int nv = wave_sum_i32(valid);
         if (lane_id() == 0 && nv) atomicAdd(&blkValid, (uint32_t)nv);
+        // list coherence: how often a valid triangle's culling box overlaps
+        // the next triangle's (the next lane's; the wave's last lane has none)
+        const float nx0 = __shfl_down(mn.x, 1, 64), ny0 = __shfl_down(mn.y, 1, 64);
+        const float nx1 = __shfl_down(mx.x, 1, 64), ny1 = __shfl_down(mx.y, 1, 64);
+        const int nvld = __shfl_down(valid, 1, 64);
+        const bool pair = valid && nvld && lane_id() < 63;
+        const bool over = pair && mn.x <= nx1 && nx0 <= mx.x && mn.y <= ny1 && ny0 <= mx.y;
+        const int np = __popcll(__ballot(pair)), no = __popcll(__ballot(over));
+        if (lane_id() == 0 && np) {
+            atomicAdd(&blkPairs, (uint32_t)np);
+            atomicAdd(&blkOverlaps, (uint32_t)no);
+        }
     }
     // count pass
     TriClamp c = tri_clamp(mn, mx, s);
@@ -796,7 +808,11 @@ __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
         a.triRange[t] = make_uint2((uint32_t)tsx | ((uint32_t)tsy << 16), (uint32_t)tex | ((uint32_t)tey << 16));
     }
     __syncthreads();
-    if (threadIdx.x == 0) a.part[blk] = blkValid;   // totalled in k_post_pairs
+    if (threadIdx.x == 0) {   // totalled in k_post_pairs
+        a.part[blk] = blkValid;
+        a.part[a.nblkT + blk] = blkPairs;
+        a.part[2 * a.nblkT + blk] = blkOverlaps;
+    }
     for (int k = threadIdx.x; k < nb; k += blockDim.x) a.histT[(size_t)k * a.nblkT + blk] = thist[k];
 }
 
@@ -1168,20 +1184,39 @@ __device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) 
     if (s->status != FD_EXIT_OK) return;
     // valid triangles: the pair-count blocks' totals (no count pass without
     // triangles)
-    __shared__ int wsum[FD_FIN_THREADS / 64];
-    int v = 0;
+    __shared__ int wsum[3][FD_FIN_THREADS / 64];
+    int v = 0, np = 0, no = 0;
     if (a.ntris > 0)
-        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) v += (int)a.part[i];
+        for (int i = threadIdx.x; i < a.nblkT; i += blockDim.x) {
+            v += (int)a.part[i];
+            np += (int)a.part[a.nblkT + i];
+            no += (int)a.part[2 * a.nblkT + i];
+        }
     v = wave_sum_i32(v);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    np = wave_sum_i32(np);
+    no = wave_sum_i32(no);
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][threadIdx.x >> 6] = v;
+        wsum[1][threadIdx.x >> 6] = np;
+        wsum[2][threadIdx.x >> 6] = no;
+    }
     __syncthreads();
     if (threadIdx.x != 0) return;
     int nv = 0;
-    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) nv += wsum[j];
+    long pairsAdj = 0, overlaps = 0;
+    for (int j = 0; j < (int)(blockDim.x + 63) >> 6; ++j) {
+        nv += wsum[0][j];
+        pairsAdj += wsum[1][j];
+        overlaps += wsum[2][j];
+    }
     s->validTris = nv;
-    // walk order of the bins' lists: largest projected triangles first
-    // unless the caller asked for the reference's order
-    s->triOrder = a.triOrder == FD_TRI_ORDER_LIST ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_AREA;
+    // walk order of the bins' lists: the reference's (ascending id) when
+    // neighbouring triangles mostly overlap (a connected mesh: its objects'
+    // triangles stay together, and a receiver usually fills from one
+    // object), else largest projected triangles first (a triangle soup: big
+    // occluders fill receivers early)
+    s->triOrder = a.triOrder != FD_TRI_ORDER_AUTO ? a.triOrder
+                  : (2 * overlaps >= pairsAdj ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_AREA);
     // valid receivers: binned (keys < nb, the scanned histogram at key nb)
     // plus keys out of range
     s->validSamples = (int)a.histR[(size_t)s->nb * a.nblkR] + s->keyOutOfRange;

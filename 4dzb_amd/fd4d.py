@@ -59,13 +59,13 @@ class Instance(ctypes.Structure):
     _fields_ = [("d_position4", ctypes.c_void_p), ("d_normal4", ctypes.c_void_p), ("width", ctypes.c_int),
                 ("height", ctypes.c_int), ("light", AreaLight), ("d_shadow", ctypes.c_void_p),
                 ("d_masks", ctypes.c_void_p), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
-                ("max_pairs", ctypes.c_ulonglong), ("occ_waves", ctypes.c_int), ("pad", ctypes.c_int)]
+                ("max_pairs", ctypes.c_ulonglong), ("occ_waves", ctypes.c_int), ("tri_order", ctypes.c_int)]
 
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("light_res", ctypes.c_int), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
-                ("ms", ctypes.c_float), ("occ_waves", ctypes.c_int)]
+                ("ms", ctypes.c_float), ("occ_waves", ctypes.c_int), ("tri_order", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -345,8 +345,9 @@ class Context:
 
     def autotune(self, inst_item, candidates, reps=3):
         """Autotuner (fd_autotune): inst_item as a Batch item (its outputs are
-        not written); candidates: list of (bins, tile_width) or (bins,
-        tile_width, waves per occlusion work item).  Returns (best index or
+        not written); candidates: list of (bins, tile_width), (bins,
+        tile_width, waves per occlusion work item) or (bins, tile_width,
+        waves, walk order: TRI_ORDER_*).  Returns (best index or
         -1, [ms per candidate: -1 = not bit-exact, -2 = its pairs do not fit
         the instance's pair arena])."""
         it = inst_item
@@ -356,7 +357,8 @@ class Context:
         for k, c in enumerate(candidates):
             b, t = c[0], c[1]
             w = c[2] if len(c) > 2 else 0
-            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, w)
+            o = c[3] if len(c) > 3 else 0
+            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, w, o)
         best = ctypes.c_int(-1)
         _check(lib().fd_autotune(self.h, ctypes.byref(inst), ctypes.cast(arr, ctypes.c_void_p), len(candidates),
                                  reps, ctypes.byref(best)), "fd_autotune")
@@ -476,13 +478,14 @@ class Batch:
 
     def __init__(self, ctx, items):
         """items: list of dicts with keys pos, nrm (DeviceBuffer), w, h, light
-        (LightDesc), shadow, masks (DeviceBuffer or None), bins, tile and
-        waves (0 = the context's setting), max_pairs."""
+        (LightDesc), shadow, masks (DeviceBuffer or None), bins, tile,
+        waves and order (0 = the context's setting), max_pairs."""
         arr = (Instance * len(items))()
         for k, it in enumerate(items):
             arr[k] = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
                               it["shadow"].ptr, it["masks"].ptr if it.get("masks") is not None else 0,
-                              it.get("bins", 0), it.get("tile", 0), it.get("max_pairs", 0), it.get("waves", 0), 0)
+                              it.get("bins", 0), it.get("tile", 0), it.get("max_pairs", 0), it.get("waves", 0),
+                              it.get("order", 0))
         h = ctypes.c_void_p()
         _check(lib().fd_batch_create(ctx.h, ctypes.cast(arr, ctypes.c_void_p), len(items), ctypes.byref(h)),
                "fd_batch_create")
@@ -518,13 +521,14 @@ class Server:
 
     def __init__(self, budget_bytes, streams=4, max_batch=16, bins=16, cull=1, eps=1e-3, max_pairs=0, tile=8,
                  tune=None):
-        """tune: optional list of dicts (scene, w, h, res, bins, tile) -- the
-        per-class layouts picked by the autotuner."""
+        """tune: optional list of dicts (scene, w, h, res, bins, tile, and
+        optionally waves and order) -- the per-class layouts picked by the
+        autotuner."""
         tune = tune or []
         self._tune = (TuneEntry * max(1, len(tune)))()
         for k, t in enumerate(tune):
             self._tune[k] = TuneEntry(t.get("scene", -1), t["w"], t["h"], t["res"], t["bins"], t["tile"],
-                                      float(t.get("ms", 0.0)), t.get("waves", 0))
+                                      float(t.get("ms", 0.0)), t.get("waves", 0), t.get("order", 0))
         cfg = ServerConfig(budget_bytes, streams, max_batch, bins, cull, eps, tile, max_pairs,
                            ctypes.cast(self._tune, ctypes.c_void_p).value if tune else None, len(tune), 0)
         h = ctypes.c_void_p()

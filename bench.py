@@ -272,13 +272,15 @@ def gpu_batched(pkg, reps=3):
         t.append(time.perf_counter() - a)
     plain_async = min(t)
     # autotuner: per class (receiver size, light grid) the fastest bit-exact
-    # layout (bins x tile width), measured on this device (untimed)
+    # layout (bins x tile width x waves) and walk order, measured on this
+    # device (untimed)
     at = pkg.autotune
     t_tune = time.perf_counter()
-    cache = at.tune_classes(ctx, 0, items)
+    cache = at.tune_classes(ctx, 0, items, orders=True)
     t_tune = time.perf_counter() - t_tune
     for it in items:
         it["bins"], it["tile"], it["waves"] = at.layout(cache, 0, it["w"], it["h"], it["light"].res_w)
+        it["order"] = at.order(cache, 0, it["w"], it["h"], it["light"].res_w)
     # batched, graph-captured
     b = fd.Batch(ctx, items)
     b.capture(strm.handle)
@@ -340,8 +342,9 @@ def gpu_batched(pkg, reps=3):
                 speedup_vs_plain_async=round(plain_async / batched, 3), masks_bit_exact_vs_plain=bool(exact),
                 per_class=per_class,
                 autotune=dict(seconds=round(t_tune, 2), classes={k: dict(bins=v["bins"], tile=v["tile"],
-                                                                        waves=v.get("waves"), ms=v["ms"],
-                                                                        default_ms=v["default_ms"])
+                                                                        waves=v.get("waves"),
+                                                                        order={1: "list", 2: "area"}.get(v.get("order"), "auto"),
+                                                                        ms=v["ms"], default_ms=v["default_ms"])
                                                                for k, v in sorted(cache.items())}))
 
 

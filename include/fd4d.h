@@ -147,10 +147,13 @@ int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
  *   FD_TRI_ORDER_AREA: by the area of the triangle projected on the light
  *     plane in 16 powers of two, largest first, ascending id within a
  *     class;
- *   FD_TRI_ORDER_AUTO (default): the area order (measured faster for
- *     triangle soups and connected meshes alike).
- * The batched and serving forms use the context's / the default setting;
- * fd_debug_read("pair_tri") always returns the reference's order. */
+ *   FD_TRI_ORDER_AUTO (default): decided per frame on the device -- the
+ *     list order when consecutive triangles mostly overlap on the light
+ *     plane (a connected mesh), else the area order (a triangle soup).
+ * The batched form takes it per instance (fd_instance.tri_order, else the
+ * context's setting), the serving form per class (fd_tune_entry.tri_order,
+ * else automatic); fd_debug_read("pair_tri") always returns the reference's
+ * order. */
 #define FD_TRI_ORDER_AUTO 0
 #define FD_TRI_ORDER_LIST 1
 #define FD_TRI_ORDER_AREA 2
@@ -231,7 +234,8 @@ typedef struct fd_instance {
     int tile_width;             /* 0 = the context's setting (fd_set_tile_width) */
     unsigned long long max_pairs; /* pair arena of this instance; 0 = 16M */
     int occ_waves;              /* 0 = the context's setting (fd_set_occlusion_waves) */
-    int pad;
+    int tri_order;              /* 0 = the context's setting (fd_set_triangle_order), else
+                                   FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA */
 } fd_instance;
 typedef struct fd_batch fd_batch;
 int fd_batch_create(fd_ctx* ctx, const fd_instance* instances, int n, fd_batch** out);
@@ -260,6 +264,7 @@ typedef struct fd_tune_entry {
     int tile_width;
     float ms;                 /* measured frame time; < 0: not bit-exact */
     int occ_waves;            /* waves per occlusion work item: 2, 4 or 8 (0 = default) */
+    int tri_order;            /* walk order: FD_TRI_ORDER_LIST / _AREA (0 = automatic) */
 } fd_tune_entry;
 typedef struct fd_server_config {
     size_t hbm_budget_bytes;  /* workspace pool (>= 64 MiB) */
@@ -316,7 +321,7 @@ int fd_server_destroy(fd_server* srv);
 
 /* No counterpart: the autotuner.  Runs `inst` over the context's scene once
  * per candidate layout (cands[i].bins_per_side, cands[i].tile_width,
- * cands[i].occ_waves), timing
+ * cands[i].occ_waves, cands[i].tri_order), timing
  * `reps` launches on the device, and compares every output bit with the
  * default layout (16x16 bins, 8-wide tiles).  cands[i].ms receives the median
  * frame time, -1 when the candidate's shadow values or masks differ (the bin

@@ -1,6 +1,8 @@
 """Batched form: every instance's result is the same alone (fd_launch) and
 in the mix (one launch per stage for all instances, direct and replayed from
 a HIP graph), bit for bit (MI355X only)."""
+import importlib
+
 import numpy as np
 import pytest
 
@@ -138,10 +140,17 @@ def test_tile_widths_and_autotune(fd, wl, oracle):
     frame.tile_width = 8
     fr = fd.FrameRunner(frame)
     item = dict(pos=fr.pos, nrm=fr.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=fr.shadow)
-    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64), (16, 8, 2), (32, 16, 2)]
+    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64), (16, 8, 2), (32, 16, 2), (16, 8, 0, 1), (16, 8, 4, 2)]
     best, ms = fr.ctx.autotune(item, cands, reps=2)
     assert 0 <= best < len(cands) and ms[best] > 0
     assert ms[1] > 0   # the reference layout itself is always exact
     assert all(m in (-1, -2) or m > 0 for m in ms)
     assert ms[best] == min(m for m in ms if m > 0)
+    assert ms[7] > 0 and ms[8] > 0   # the walk orders change no bit
+    at = importlib.import_module("4dzb_amd.autotune")
+    e = at.tune_one(fr.ctx, item, [(16, 8, 4), (8, 8, 4)], reps=2, order_items=[item, item])
+    assert e["order"] in (1, 2) and e["ms"] > 0
+    assert e["order_ms"]["instances"] == 2 and e["order_ms"]["list"] > 0 and e["order_ms"]["area"] > 0
+    assert (e["order"] == 2) == (e["order_ms"]["area"] < 0.97 * e["order_ms"]["list"])
+    assert at.tune_one(fr.ctx, item, [(16, 8, 4)], reps=1)["order"] == 0   # no order choice asked: automatic
     fr.close()

@@ -1,12 +1,13 @@
 """Walk order of the bins' triangle lists in the occlusion (MI355X only).
 
 fd_set_triangle_order picks the order in which the occlusion kernel walks
-each bin's list: the reference's (ascending triangle id) or the largest
-triangles on the light plane first (16 power-of-two area classes, the
-default).  A receiver's mask is the OR of its triangles' masks, so the order must not
+each bin's list: the reference's (ascending triangle id), the largest
+triangles on the light plane first (16 power-of-two area classes), or per
+frame on the device (the list order for connected meshes, the area order
+for triangle soups).  A receiver's mask is the OR of its triangles' masks, so the order must not
 change a bit of any output; it only changes how soon receivers fill up.
 
-Checked here: the default choice, oracle bits under every order, the
+Checked here: the automatic choice, oracle bits under every order, the
 area-ordered lists as a stable sort of the reference's lists by the
 triangles' area classes (restated here in float32, and checked against their
 definition), and the batched form over a soup.
@@ -53,7 +54,7 @@ def _soup_small(wl):
     return wl.boxes_1080p(w=160, h=90, ntri=60_000, light_res=40)
 
 
-@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", AREA)])
+@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", LIST)])
 def test_auto_choice_and_oracle(fd, wl, oracle, which, expect):
     frame = {"soup": lambda: _soup(wl), "soup32": lambda: _soup(wl, 32), "soup-small": lambda: _soup_small(wl),
              "mesh": lambda: _mesh(wl)}[which]()
@@ -194,5 +195,34 @@ def test_batch_over_soup(fd, wl):
         mk = it["masks"].download(np.uint32)
         assert _digest(sh, mk) == alone[k], k
     b.close()
+    # per-instance orders in one batch: the same bits
+    print("[order] batch with per-instance orders", flush=True)
+    for it, order in zip(items, (LIST, AREA, LIST)):
+        it["order"] = order
+        it["shadow"].upload(np.zeros(it["shadow"].nbytes, np.uint8))
+        it["masks"].upload(np.full(it["masks"].nbytes, 0xA5, np.uint8))
+    b = fd.Batch(ctx, items)
+    b.launch(stream.handle)
+    stream.sync()
+    for k, it in enumerate(items):
+        rc, st = b.stats(k)
+        assert rc == 0 and st.exit_code == 0, k
+        assert _digest(it["shadow"].download(np.float32), it["masks"].download(np.uint32)) == alone[k], k
+    b.close()
+    # per-class orders from a server's tuning table: the same bits
+    print("[order] server with per-class orders", flush=True)
+    tune = [dict(scene=0, w=it["w"], h=it["h"], res=it["light"].res_w, bins=16, tile=8, order=order, ms=1.0)
+            for it, order in zip(items, (AREA, LIST, AREA))]
+    srv = fd.Server(budget_bytes=1 << 30, streams=2, max_batch=2, eps=1e-3, tune=tune)
+    srv.add_scene(verts, scene.nverts, 16, idx, scene.ntris)
+    for k, it in enumerate(items):
+        it["masks"].upload(np.full(it["masks"].nbytes, 0x5A, np.uint8))
+        srv.submit(0, it["pos"], it["nrm"], it["w"], it["h"], it["light"], it["shadow"], it["masks"], tag=k)
+    srv.drain(120000)
+    done = srv.poll(len(items) + 2)
+    srv.close()
+    assert len(done) == len(items) and all(c.status == 0 for c in done)
+    for k, it in enumerate(items):
+        assert _digest(it["shadow"].download(np.float32), it["masks"].download(np.uint32)) == alone[k], k
     stream.close()
     ctx.close()

@@ -52,7 +52,8 @@ def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None):
     """Tune one instance class on `ctx`'s scene.  item: a Batch item (pos,
     nrm, w, h, light, shadow).  order_items: instances of the class on which
     to time the two walk orders at the chosen layout (None: no order
-    choice, the automatic one).  Returns the cache entry."""
+    choice, the automatic one); the entry's ms is then the first of them
+    under the chosen order.  Returns the cache entry."""
     t0 = time.perf_counter()
     best, ms = ctx.autotune(item, candidates, reps)
     dl = default_layout(item["light"].res_w)
@@ -63,15 +64,18 @@ def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None):
     order, order_ms = 0, {}
     if order_items:
         tl = ta = 0.0
+        first = None
         for it in order_items:
             _, oms = ctx.autotune(it, [(b, t, w, ORDER_LIST), (b, t, w, ORDER_AREA)], reps)
             if min(oms) < 0:   # not exact or no fit (cannot happen: the order changes no bit)
                 tl = ta = -1.0
                 break
+            first = first or oms
             tl, ta = tl + oms[0], ta + oms[1]
         order_ms = {"list": round(tl, 4), "area": round(ta, 4), "instances": len(order_items)}
         if tl > 0 and ta > 0:
             order = ORDER_AREA if ta < 0.97 * tl else ORDER_LIST
+            best_ms = first[order - 1]
     return dict(bins=b, tile=t, waves=w, order=order, ms=best_ms, default_ms=default,
                 candidates={"x".join(str(x) for x in c[:3]): (round(m, 4) if m >= 0 else ("not exact" if m == -1 else "pair arena too small"))
                             for c, m in zip(candidates, ms)},

@@ -27,3 +27,58 @@ def test_cache_and_tables(tmp_path):
     assert tab[1]["order"] == 0
     assert tab[1]["w"] == 1920 and tab[1]["h"] == 1080 and tab[1]["bins"] == 8
     assert not os.path.exists(p + ".tmp")
+
+
+class _FakeCtx:
+    """Stands in for fd4d.Context.autotune: layout candidates get fixed
+    times; walk-order pairs get the list / area times of the instance."""
+
+    def __init__(self, layout_ms, order_ms):
+        self.layout_ms = layout_ms
+        self.order_ms = order_ms
+        self.order_calls = []
+
+    def autotune(self, item, cands, reps=3):
+        if len(cands[0]) == 4:   # (bins, tile, waves, order) pairs
+            self.order_calls.append((item["id"], tuple(cands)))
+            ms = list(self.order_ms[item["id"]])
+        else:
+            ms = [self.layout_ms.get(tuple(c), -1.0) for c in cands]
+        ok = [m for m in ms if m >= 0]
+        return (ms.index(min(ok)) if ok else -1), ms
+
+
+class _Light:
+    res_w = 16
+
+
+def _items(n, w=512):
+    return [dict(id=k, w=w, h=w, light=_Light()) for k in range(n)]
+
+
+def test_order_choice_over_class_instances():
+    at = importlib.import_module("4dzb_amd.autotune")
+    cands = [(16, 8, 4), (8, 8, 4)]
+    layout_ms = {(16, 8, 4): 2.0, (8, 8, 4): 1.5}
+    # area faster on the first instances, slower in sum: the class keeps the list order
+    order_ms = {0: (1.0, 0.8), 1: (1.0, 0.8), 2: (1.0, 0.8), 3: (1.0, 2.0)}
+    ctx = _FakeCtx(layout_ms, order_ms)
+    cache = at.tune_classes(ctx, 0, _items(4), candidates=cands, orders=True)
+    e = cache[at.class_key(0, 512, 512, 16)]
+    assert (e["bins"], e["tile"], e["waves"]) == (8, 8, 4)
+    assert e["order_ms"] == {"list": 4.0, "area": 4.4, "instances": 4} and e["order"] == at.ORDER_LIST
+    assert e["ms"] == 1.0   # the order pair's time of the class's representative
+    # every order pair is timed at the chosen layout
+    assert all(c == ((8, 8, 4, at.ORDER_LIST), (8, 8, 4, at.ORDER_AREA)) for _, c in ctx.order_calls)
+    # area 3% or more faster in sum: area; less than 3%: list
+    for area, want in ((0.969, at.ORDER_AREA), (0.975, at.ORDER_LIST)):
+        ctx = _FakeCtx(layout_ms, {k: (1.0, area) for k in range(12)})
+        e = at.tune_one(ctx, _items(1)[0], cands, order_items=_items(12)[:at.ORDER_SAMPLES])
+        assert e["order"] == want and e["order_ms"]["instances"] == at.ORDER_SAMPLES
+    # at most ORDER_SAMPLES instances per class decide; without orders=True, no order pairs run
+    ctx = _FakeCtx(layout_ms, {k: (1.0, 0.5) for k in range(12)})
+    at.tune_classes(ctx, 0, _items(12), candidates=cands, orders=True)
+    assert sorted(i for i, _ in ctx.order_calls) == list(range(at.ORDER_SAMPLES))
+    ctx = _FakeCtx(layout_ms, {})
+    cache = at.tune_classes(ctx, 0, _items(3), candidates=cands)
+    assert not ctx.order_calls and cache[at.class_key(0, 512, 512, 16)]["order"] == 0

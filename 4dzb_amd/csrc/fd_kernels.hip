@@ -659,7 +659,9 @@ __device__ __forceinline__ int area_class(const float (&px)[3], const float (&py
 }
 
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
-    extern __shared__ uint32_t thist[];  // nb counters, then the staged bin statistics
+    // nb counters, then the staged bin statistics (float4 + float per bin);
+    // 16-byte aligned behind the block's static counters, for the float4s
+    extern __shared__ __align__(16) uint32_t thist[];
     __shared__ uint32_t blkValid, blkPairs, blkOverlaps;
     const FdState* s = a.st;
     if (s->status != FD_EXIT_OK) return;
@@ -1734,7 +1736,9 @@ __device__ __forceinline__ void cta_tris(const FdArgs& a, uint32_t* lds, T* full
 // merged with LDS atomics), so a CU holds twice the waves for the same LDS.
 template <typename T, bool COUNT, bool CTA = false, int NW = 1>
 __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
-    extern __shared__ uint32_t lds[];  // wpr * R * 64 row words, then 64 full-row masks (T)
+    // wpr * R * 64 row words, then 64 full-row masks (T, 64-bit atomics for
+    // 64-bit rows: the array is aligned for them)
+    extern __shared__ __align__(16) uint32_t lds[];
     const FdState* s = a.st;
     const int status = s->status;
     const int R = a.L.res;                  // rows == columns

@@ -12,7 +12,10 @@ ORDER_SAMPLES instances of the class, and the area order is kept when it is at
 least 3% faster over them (the automatic choice per frame keeps the
 reference's order for connected meshes, where the area order wins or loses
 by up to 40% depending on the light pose, so one instance is not enough to
-decide a class).  The bin grid
+decide a class).  Optionally (splits=True) the number of stretches per bin
+triangle list is then timed at that layout and order -- the library's
+automatic rule against fixed counts -- and a fixed count is kept when it is
+at least 3% faster.  The bin grid
 also sets the receiver sort's key width (one counting-sort pass over
 2*log2(bins) + 1 bits), so picking the grid picks the sort's radix width.
 
@@ -32,6 +35,7 @@ WAVES = (4, 8)
 CANDIDATES = [(b, t, w) for w in WAVES for b in BINS for t in TILES]
 ORDER_LIST, ORDER_AREA = 1, 2   # FD_TRI_ORDER_LIST / FD_TRI_ORDER_AREA
 ORDER_SAMPLES = 8               # instances of a class that decide its walk order
+SPLITS = (1, 2, 4, 8, 16)       # stretches per bin list tried against the automatic rule (0)
 
 
 def default_waves(res):
@@ -48,12 +52,14 @@ def class_key(scene, w, h, res):
     return f"{scene}:{w}x{h}:{res}"
 
 
-def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None):
+def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None, splits=None):
     """Tune one instance class on `ctx`'s scene.  item: a Batch item (pos,
     nrm, w, h, light, shadow).  order_items: instances of the class on which
     to time the two walk orders at the chosen layout (None: no order
     choice, the automatic one); the entry's ms is then the first of them
-    under the chosen order.  Returns the cache entry."""
+    under the chosen order.  splits: stretch counts to time against the
+    automatic rule on item (None: the automatic rule).  Returns the cache
+    entry."""
     t0 = time.perf_counter()
     best, ms = ctx.autotune(item, candidates, reps)
     dl = default_layout(item["light"].res_w)
@@ -76,14 +82,23 @@ def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None):
         if tl > 0 and ta > 0:
             order = ORDER_AREA if ta < 0.97 * tl else ORDER_LIST
             best_ms = first[order - 1]
-    return dict(bins=b, tile=t, waves=w, order=order, ms=best_ms, default_ms=default,
+    # stretches per bin list at that layout and order
+    split, split_ms = 0, {}
+    if splits:
+        cands = [(b, t, w, order, 0)] + [(b, t, w, order, sp) for sp in splits]
+        sb, sms = ctx.autotune(item, cands, reps)
+        split_ms = {str(c[4]): (round(m, 4) if m >= 0 else None) for c, m in zip(cands, sms)}
+        if sb > 0 and sms[0] > 0 and sms[sb] < 0.97 * sms[0]:
+            split = cands[sb][4]
+            best_ms = sms[sb]
+    return dict(bins=b, tile=t, waves=w, order=order, split=split, ms=best_ms, default_ms=default,
                 candidates={"x".join(str(x) for x in c[:3]): (round(m, 4) if m >= 0 else ("not exact" if m == -1 else "pair arena too small"))
                             for c, m in zip(candidates, ms)},
-                order_ms=order_ms,
+                order_ms=order_ms, split_ms=split_ms,
                 tune_s=round(time.perf_counter() - t0, 3))
 
 
-def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3, orders=False):
+def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3, orders=False, splits=False):
     """items: Batch items; one representative per class (scene, w, h, res) is
     tuned unless the cache already holds it (with orders, up to
     ORDER_SAMPLES of the class's items decide its walk order).  Returns the
@@ -95,7 +110,8 @@ def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3, o
     for key, its in groups.items():
         if key in cache:
             continue
-        cache[key] = tune_one(ctx, its[0], candidates, reps, order_items=its[:ORDER_SAMPLES] if orders else None)
+        cache[key] = tune_one(ctx, its[0], candidates, reps, order_items=its[:ORDER_SAMPLES] if orders else None,
+                              splits=SPLITS if splits else None)
     return cache
 
 
@@ -111,6 +127,12 @@ def order(cache, scene, w, h, res):
     return e.get("order", 0) if e else 0
 
 
+def split(cache, scene, w, h, res):
+    """Stretches per bin list of a class, 0 = the automatic rule."""
+    e = cache.get(class_key(scene, w, h, res))
+    return e.get("split", 0) if e else 0
+
+
 def server_table(cache):
     """The cache as the server's per-class table (fd_server_config.tune)."""
     out = []
@@ -118,7 +140,8 @@ def server_table(cache):
         scene, wh, res = key.split(":")
         w, h = wh.split("x")
         out.append(dict(scene=int(scene), w=int(w), h=int(h), res=int(res), bins=e["bins"], tile=e["tile"],
-                        waves=e.get("waves", default_waves(int(res))), order=e.get("order", 0), ms=e["ms"] or 0.0))
+                        waves=e.get("waves", default_waves(int(res))), order=e.get("order", 0),
+                        split=e.get("split", 0), ms=e["ms"] or 0.0))
     return out
 
 

@@ -93,6 +93,7 @@ struct fd_ctx {
     int occVariant = FD_OCC_PER_SAMPLE;
     int triOrder = FD_TRI_ORDER_AUTO;
     int occWaves = 0;     // 0: fd_default_waves of the frame's row width
+    int listSplit = 0;    // 0: the automatic rule
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
     std::vector<Model> models;
@@ -155,14 +156,29 @@ void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm,
     a.nT = (long)nb * a.nblkT;
     a.nbR = fdk::reduce_blocks(a.WH);
     a.nbV = fdk::reduce_blocks(a.nverts);
-    // small frames (up to 2048 work items: two waves per SIMD) leave most of
-    // the GPU idle with one wave per 64 receivers, each walking its bin's
-    // whole triangle list: split the lists to about 16k work items.  Larger
-    // frames fill the GPU already, and splitting would only cost the merge and
-    // the early exit on full rows.
     a.chunkSlots = cdiv(a.WH, 64) + nb + 1;
-    a.segs = (md.ntris > 0 && a.chunkSlots <= 2048) ? std::max(1, std::min(16, 16384 / a.chunkSlots)) : 1;
+    a.segs = 1;   // apply_split, once the waves per work item are known
+    a.splitReq = 0;
     a.binLds = nb <= FD_LDS_BINS;
+}
+
+// Stretches per bin list: req (1..FD_LIST_SPLIT_MAX), or for 0 the
+// automatic rule.  Small frames (up to FD_SPLIT_ITEMS work items) leave
+// most of the GPU idle while each work item walks its bin's whole list:
+// their lists split into about FD_SPLIT_WAVES waves in all.  Larger frames
+// fill the GPU already, and splitting would only cost the merge and the
+// early exit on full rows.  A frame without triangles has no lists to
+// split.  Needs a.occWaves.
+void apply_split(FdArgs& a, int req) {
+    a.splitReq = req > 0 ? req : 0;
+    if (a.ntris <= 0) {
+        a.segs = 1;
+    } else if (req > 0) {
+        a.segs = std::min(req, FD_LIST_SPLIT_MAX);
+    } else {
+        const long waves = (long)a.chunkSlots * std::max(a.occWaves, 1);
+        a.segs = a.chunkSlots <= FD_SPLIT_ITEMS ? (int)std::max(1L, std::min(16L, FD_SPLIT_WAVES / waves)) : 1;
+    }
 }
 
 // Place the frame's workspace at `base` (nullptr: size only); returns bytes.
@@ -224,6 +240,7 @@ int frame_args(fd_ctx* c, FdArgs& a) {
     a.occVariant = c->occVariant;
     a.triOrder = c->triOrder;
     a.occWaves = c->occWaves ? c->occWaves : fd_default_waves(a.L.wpr);
+    apply_split(a, c->listSplit);
     return FD_OK;
 }
 
@@ -374,6 +391,12 @@ int fd_set_backface_cull(fd_ctx* c, int enable) {
 int fd_set_occlusion_variant(fd_ctx* c, int variant) {
     if (!c || (variant != FD_OCC_PER_SAMPLE && variant != FD_OCC_BIN_QUEUE)) return FD_ERR_ARG;
     c->occVariant = variant;
+    return FD_OK;
+}
+
+int fd_set_list_split(fd_ctx* c, int segs) {
+    if (!c || segs < 0 || segs > FD_LIST_SPLIT_MAX) return FD_ERR_ARG;
+    c->listSplit = segs;
     return FD_OK;
 }
 
@@ -700,7 +723,7 @@ size_t build_plan(const std::vector<FdArgs>& args, bool anyTris, char* devBase, 
     for (int i = 0; i < n; ++i) slots += args[i].chunkSlots;
     for (int i = 0; i < n; ++i) {
         hArgs[i] = args[i];
-        if (slots > 2048) hArgs[i].segs = 1;
+        if (slots > FD_SPLIT_ITEMS && !hArgs[i].splitReq) hArgs[i].segs = 1;
         hArgs[i].binLds = P.maxNb <= FD_LDS_BINS;   // the batched launch sizes LDS for the largest grid
     }
     int k = n;
@@ -800,11 +823,13 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
             delete b;
             return FD_ERR_ARG;
         }
+        if (q.list_split < 0 || q.list_split > FD_LIST_SPLIT_MAX) { delete b; return FD_ERR_ARG; }
         const unsigned long long cap = q.max_pairs ? q.max_pairs : (16ull << 20);
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
                   c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
         b->args[i].triOrder = q.tri_order ? q.tri_order : c->triOrder;
         b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves ? c->occWaves : fd_default_waves(L.wpr);
+        apply_split(b->args[i], q.list_split ? q.list_split : c->listSplit);   // after occWaves
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);
     }
@@ -893,12 +918,13 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
     // one layout: returns a status, the median device time and the outputs
-    auto run = [&](int bins, int tw, int waves, int order, uint32_t* dst, float* ms) -> int {
+    auto run = [&](int bins, int tw, int waves, int order, int split, uint32_t* dst, float* ms) -> int {
         fd_instance q = *inst;
         q.bins_per_side = bins;
         q.tile_width = tw;
         q.occ_waves = waves;
         q.tri_order = order;
+        q.list_split = split;
         q.d_shadow = (float*)dst;
         q.d_masks = dst + WH;
         fd_batch* b = nullptr;
@@ -926,12 +952,12 @@ extern "C" int fd_autotune(fd_ctx* c, const fd_instance* inst, fd_tune_entry* ca
     };
     int rc = hip_status(e);
     float refMs = 0.0f;
-    if (rc == FD_OK) rc = run(16, 8, 0, 0, (uint32_t*)ref.p, &refMs);
+    if (rc == FD_OK) rc = run(16, 8, 0, 0, 0, (uint32_t*)ref.p, &refMs);
     *best = -1;
     for (int i = 0; i < n && rc == FD_OK; ++i) {
         float ms = 0.0f;
-        rc = run(cands[i].bins_per_side, cands[i].tile_width, cands[i].occ_waves, cands[i].tri_order, (uint32_t*)out.p,
-                 &ms);
+        rc = run(cands[i].bins_per_side, cands[i].tile_width, cands[i].occ_waves, cands[i].tri_order,
+                 cands[i].list_split, (uint32_t*)out.p, &ms);
         if (rc == FD_ERR_CAPACITY) {   // this grid needs more pairs than the instance's arena
             cands[i].ms = -2.0f;
             rc = FD_OK;

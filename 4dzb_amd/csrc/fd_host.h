@@ -41,6 +41,8 @@ void light_setup(const fd_area_light& d, FdLight& L);
 void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm, int W, int H, const FdLight& L,
                float eps, int cull, int gridLog2, int tileLog2, float* shadow, uint32_t* masks,
                unsigned long long pairCap);
+// an explicit list split (fd_set_list_split), 0 = the automatic rule
+void apply_split(FdArgs& a, int req);
 // place the frame workspace at base (nullptr: size only); returns bytes
 size_t layout(FdArgs& a, char* base, bool countOps);
 void fill_stats(const FdState& s, fd_stats* o);

@@ -80,7 +80,19 @@ struct FdArgs {
     int triOrder;
     // waves per occlusion work item (per-sample kernel): 2, 4 or 8
     int occWaves;
+    // stretches per bin list asked for (fd_set_list_split; 0 = the automatic
+    // rule, which a batch that fills the GPU overrides with no split)
+    int splitReq;
 };
+
+// Stretches of a bin's triangle list (fd_set_list_split): 1 = whole lists,
+// up to FD_LIST_SPLIT_MAX (fd4d.h).  The automatic rule: frames of up to
+// FD_SPLIT_ITEMS work items split their lists into about FD_SPLIT_WAVES
+// waves in all (work items x waves per item), at most 16 stretches.  Each
+// stretch merges its masks into the output by atomic OR, so the more mask
+// words per receiver, the less splitting pays.
+#define FD_SPLIT_ITEMS 2048
+#define FD_SPLIT_WAVES 20480
 
 // Waves per occlusion work item (per-sample kernel) unless a frame asks for
 // another count (2, 4 or 8): the waves share a work item's receivers, LDS

@@ -302,7 +302,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     light_setup(r->light, L);
     // layout of this request's class: the tuning table, else the defaults
     int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width, waves = fd_default_waves(L.wpr);
-    int order = FD_TRI_ORDER_AUTO;
+    int order = FD_TRI_ORDER_AUTO, split = 0;
     for (const fd_tune_entry& t : s->tune)
         if ((t.scene < 0 || t.scene == r->scene) && t.width == r->width && t.height == r->height &&
             t.light_res == r->light.res_w && t.ms >= 0.0f) {
@@ -310,6 +310,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
             tw = t.tile_width;
             if (fd_valid_waves(t.occ_waves)) waves = t.occ_waves;
             if (t.tri_order == FD_TRI_ORDER_LIST || t.tri_order == FD_TRI_ORDER_AREA) order = t.tri_order;
+            if (t.list_split > 0 && t.list_split <= FD_LIST_SPLIT_MAX) split = t.list_split;
             break;
         }
     int g = 0, tl = 0;
@@ -320,6 +321,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
               r->height, L, s->cfg.eps, s->cfg.backface_cull, g, tl, r->d_shadow, r->d_masks, s->cfg.max_pairs);
     q.args.occWaves = waves;
     q.args.triOrder = order;
+    apply_split(q.args, split);
     FdArgs probe = q.args;
     q.bytes = layout(probe, nullptr, false);
     if (q.bytes > s->pool.size()) return FD_ERR_CAPACITY;   // can never be admitted

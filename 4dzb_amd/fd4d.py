@@ -59,13 +59,15 @@ class Instance(ctypes.Structure):
     _fields_ = [("d_position4", ctypes.c_void_p), ("d_normal4", ctypes.c_void_p), ("width", ctypes.c_int),
                 ("height", ctypes.c_int), ("light", AreaLight), ("d_shadow", ctypes.c_void_p),
                 ("d_masks", ctypes.c_void_p), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
-                ("max_pairs", ctypes.c_ulonglong), ("occ_waves", ctypes.c_int), ("tri_order", ctypes.c_int)]
+                ("max_pairs", ctypes.c_ulonglong), ("occ_waves", ctypes.c_int), ("tri_order", ctypes.c_int),
+                ("list_split", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("light_res", ctypes.c_int), ("bins_per_side", ctypes.c_int), ("tile_width", ctypes.c_int),
-                ("ms", ctypes.c_float), ("occ_waves", ctypes.c_int), ("tri_order", ctypes.c_int)]
+                ("ms", ctypes.c_float), ("occ_waves", ctypes.c_int), ("tri_order", ctypes.c_int),
+                ("list_split", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -132,6 +134,7 @@ def lib():
             "fd_set_occlusion_variant": ([vp, ip], ip),
             "fd_set_triangle_order": ([vp, ip], ip),
             "fd_set_occlusion_waves": ([vp, ip], ip),
+            "fd_set_list_split": ([vp, ip], ip),
             "fd_autotune": ([vp, ctypes.c_void_p, ctypes.c_void_p, ip, ip, ctypes.POINTER(ip)], ip),
             "fd_set_view": ([vp, ip, ip], ip),
             "fd_set_scenes_esp": ([vp, ctypes.c_float], ip),
@@ -338,6 +341,11 @@ class Context:
         grids up to 32x32, 8 above).  Changes no result."""
         _check(lib().fd_set_occlusion_waves(self.h, int(waves)), f"fd_set_occlusion_waves({waves})")
 
+    def set_list_split(self, segs):
+        """Stretches per bin triangle list: 0 = the automatic rule, 1..64 that
+        many (changes no result)."""
+        _check(lib().fd_set_list_split(self.h, int(segs)), f"fd_set_list_split({segs})")
+
     def set_triangle_order(self, order):
         """TRI_ORDER_AUTO / TRI_ORDER_LIST / TRI_ORDER_AREA: the order in which
         the occlusion walks each bin's triangle list (changes no result)."""
@@ -346,19 +354,21 @@ class Context:
     def autotune(self, inst_item, candidates, reps=3):
         """Autotuner (fd_autotune): inst_item as a Batch item (its outputs are
         not written); candidates: list of (bins, tile_width), (bins,
-        tile_width, waves per occlusion work item) or (bins, tile_width,
-        waves, walk order: TRI_ORDER_*).  Returns (best index or
+        tile_width, waves per occlusion work item), (bins, tile_width,
+        waves, walk order: TRI_ORDER_*) or (bins, tile_width, waves, order,
+        list split).  Returns (best index or
         -1, [ms per candidate: -1 = not bit-exact, -2 = its pairs do not fit
         the instance's pair arena])."""
         it = inst_item
         inst = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
-                        it["shadow"].ptr, 0, 0, 0, it.get("max_pairs", 0), 0, 0)
+                        it["shadow"].ptr, 0, 0, 0, it.get("max_pairs", 0), 0, 0, 0, 0)
         arr = (TuneEntry * len(candidates))()
         for k, c in enumerate(candidates):
             b, t = c[0], c[1]
             w = c[2] if len(c) > 2 else 0
             o = c[3] if len(c) > 3 else 0
-            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, w, o)
+            sp = c[4] if len(c) > 4 else 0
+            arr[k] = TuneEntry(-1, it["w"], it["h"], it["light"].res_w, b, t, 0.0, w, o, sp)
         best = ctypes.c_int(-1)
         _check(lib().fd_autotune(self.h, ctypes.byref(inst), ctypes.cast(arr, ctypes.c_void_p), len(candidates),
                                  reps, ctypes.byref(best)), "fd_autotune")
@@ -446,6 +456,7 @@ class FrameRunner:
         c.set_bin_num(1 << frame.grid_log2)
         c.set_tile_width(frame.tile_width)
         c.set_occlusion_waves(getattr(frame, "occ_waves", 0))
+        c.set_list_split(getattr(frame, "list_split", 0))
         if pair_capacity:
             c.set_pair_capacity(pair_capacity)
         c.add_model(self.verts, sc.nverts, sc.verts.shape[1] * 4, self.idx, sc.ntris)
@@ -479,13 +490,13 @@ class Batch:
     def __init__(self, ctx, items):
         """items: list of dicts with keys pos, nrm (DeviceBuffer), w, h, light
         (LightDesc), shadow, masks (DeviceBuffer or None), bins, tile,
-        waves and order (0 = the context's setting), max_pairs."""
+        waves, order and split (0 = the context's setting), max_pairs."""
         arr = (Instance * len(items))()
         for k, it in enumerate(items):
             arr[k] = Instance(it["pos"].ptr, it["nrm"].ptr, it["w"], it["h"], AreaLight.from_desc(it["light"]),
                               it["shadow"].ptr, it["masks"].ptr if it.get("masks") is not None else 0,
                               it.get("bins", 0), it.get("tile", 0), it.get("max_pairs", 0), it.get("waves", 0),
-                              it.get("order", 0))
+                              it.get("order", 0), it.get("split", 0), 0)
         h = ctypes.c_void_p()
         _check(lib().fd_batch_create(ctx.h, ctypes.cast(arr, ctypes.c_void_p), len(items), ctypes.byref(h)),
                "fd_batch_create")
@@ -522,13 +533,14 @@ class Server:
     def __init__(self, budget_bytes, streams=4, max_batch=16, bins=16, cull=1, eps=1e-3, max_pairs=0, tile=8,
                  tune=None):
         """tune: optional list of dicts (scene, w, h, res, bins, tile, and
-        optionally waves and order) -- the per-class layouts picked by the
+        optionally waves, order and split) -- the per-class layouts picked by the
         autotuner."""
         tune = tune or []
         self._tune = (TuneEntry * max(1, len(tune)))()
         for k, t in enumerate(tune):
             self._tune[k] = TuneEntry(t.get("scene", -1), t["w"], t["h"], t["res"], t["bins"], t["tile"],
-                                      float(t.get("ms", 0.0)), t.get("waves", 0), t.get("order", 0))
+                                      float(t.get("ms", 0.0)), t.get("waves", 0), t.get("order", 0),
+                                      t.get("split", 0))
         cfg = ServerConfig(budget_bytes, streams, max_batch, bins, cull, eps, tile, max_pairs,
                            ctypes.cast(self._tune, ctypes.c_void_p).value if tune else None, len(tune), 0)
         h = ctypes.c_void_p()

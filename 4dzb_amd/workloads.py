@@ -121,7 +121,8 @@ class Frame:
     backface_cull: int = 1
     grid_log2: int = 4
     tile_width: int = 8
-    occ_waves: int = 0      # waves per occlusion work item (0: the library's default)
+    occ_waves: int = 0       # waves per occlusion work item (0: the library's default)
+    list_split: int = 0      # stretches per bin triangle list (0: the library's rule)
 
 
 def receivers_plane(w, h, half=4.0):

@@ -410,7 +410,7 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
     for k, q in enumerate(reqs):
         r, pos, nrm = rx[(q["w"], q["h"])]
         item = dict(pos=pos, nrm=nrm, w=q["w"], h=q["h"], light=lights[k], shadow=outs[k][0], max_pairs=64 << 20)
-        at.tune_classes(ctxs[q["scene"]], q["scene"], [item], cache, reps=2)
+        at.tune_classes(ctxs[q["scene"]], q["scene"], [item], cache, reps=2, splits=True)
     t_tune = time.perf_counter() - t_tune
     for c in ctxs:
         c.close()
@@ -486,7 +486,8 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
                 autotune=dict(seconds=round(t_tune, 2), classes=len(cache),
                               non_default=sum(1 for k, v in cache.items()
                                               if (v["bins"], v["tile"], v.get("waves"))
-                                              != pkg.autotune.default_layout(int(k.split(":")[2])))))
+                                              != pkg.autotune.default_layout(int(k.split(":")[2]))),
+                              split_classes=sum(1 for v in cache.values() if v.get("split"))))
 
 
 def gpu_target_config(pkg, steps=10):

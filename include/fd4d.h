@@ -167,6 +167,20 @@ int fd_set_triangle_order(fd_ctx* ctx, int order);
  * form per class (fd_tune_entry.occ_waves). */
 int fd_set_occlusion_waves(fd_ctx* ctx, int waves);
 
+/* No counterpart (a schedule knob of this implementation): stretches per
+ * bin triangle list.  The per-sample occlusion gives each 64 receivers of a
+ * bin one work item that walks the bin's whole list; with S stretches it
+ * gives them S work items, one per stretch of the list, whose masks merge by
+ * atomic OR into the (zeroed) mask output before a pass counts them.  OR
+ * does not depend on the order, so the bits are the same for every S.
+ * 0 (the default): frames of up to 2048 work items split into about 16384
+ * work items (at most 16 stretches), larger ones do not (nor does a batch
+ * whose instances total more than 2048); 1..64: that many stretches.  The
+ * batched form takes it per instance (fd_instance.list_split), the serving
+ * form per class (fd_tune_entry.list_split). */
+#define FD_LIST_SPLIT_MAX 64
+int fd_set_list_split(fd_ctx* ctx, int segs);
+
 /* FD::FDsetView, FD/api.cpp:205-237. */
 int fd_set_view(fd_ctx* ctx, int width, int height);
 
@@ -236,6 +250,8 @@ typedef struct fd_instance {
     int occ_waves;              /* 0 = the context's setting (fd_set_occlusion_waves) */
     int tri_order;              /* 0 = the context's setting (fd_set_triangle_order), else
                                    FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA */
+    int list_split;             /* 0 = the context's setting (fd_set_list_split) */
+    int pad;
 } fd_instance;
 typedef struct fd_batch fd_batch;
 int fd_batch_create(fd_ctx* ctx, const fd_instance* instances, int n, fd_batch** out);
@@ -265,6 +281,7 @@ typedef struct fd_tune_entry {
     float ms;                 /* measured frame time; < 0: not bit-exact */
     int occ_waves;            /* waves per occlusion work item: 2, 4 or 8 (0 = default) */
     int tri_order;            /* walk order: FD_TRI_ORDER_LIST / _AREA (0 = automatic) */
+    int list_split;           /* stretches per bin list (0 = automatic; fd_set_list_split) */
 } fd_tune_entry;
 typedef struct fd_server_config {
     size_t hbm_budget_bytes;  /* workspace pool (>= 64 MiB) */
@@ -321,7 +338,7 @@ int fd_server_destroy(fd_server* srv);
 
 /* No counterpart: the autotuner.  Runs `inst` over the context's scene once
  * per candidate layout (cands[i].bins_per_side, cands[i].tile_width,
- * cands[i].occ_waves, cands[i].tri_order), timing
+ * cands[i].occ_waves, cands[i].tri_order, cands[i].list_split), timing
  * `reps` launches on the device, and compares every output bit with the
  * default layout (16x16 bins, 8-wide tiles).  cands[i].ms receives the median
  * frame time, -1 when the candidate's shadow values or masks differ (the bin

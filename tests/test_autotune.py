@@ -23,7 +23,8 @@ def test_cache_and_tables(tmp_path):
     assert at.order(cache, 2, 1920, 1080, 16) == 0   # an entry without an order: automatic
     assert at.order(cache, 0, 256, 256, 32) == 0
     tab = sorted(at.server_table(cache), key=lambda t: t["scene"])
-    assert tab[0] == dict(scene=0, w=512, h=512, res=32, bins=32, tile=16, waves=2, order=at.ORDER_AREA, ms=1.5)
+    assert tab[0] == dict(scene=0, w=512, h=512, res=32, bins=32, tile=16, waves=2, order=at.ORDER_AREA, split=0,
+                          ms=1.5)
     assert tab[1]["order"] == 0
     assert tab[1]["w"] == 1920 and tab[1]["h"] == 1080 and tab[1]["bins"] == 8
     assert not os.path.exists(p + ".tmp")
@@ -33,13 +34,18 @@ class _FakeCtx:
     """Stands in for fd4d.Context.autotune: layout candidates get fixed
     times; walk-order pairs get the list / area times of the instance."""
 
-    def __init__(self, layout_ms, order_ms):
+    def __init__(self, layout_ms, order_ms, split_ms=None):
         self.layout_ms = layout_ms
         self.order_ms = order_ms
+        self.split_ms = split_ms or {}
         self.order_calls = []
+        self.split_calls = []
 
     def autotune(self, item, cands, reps=3):
-        if len(cands[0]) == 4:   # (bins, tile, waves, order) pairs
+        if len(cands[0]) == 5:   # (bins, tile, waves, order, split)
+            self.split_calls.append(tuple(cands))
+            ms = [self.split_ms[c[4]] for c in cands]
+        elif len(cands[0]) == 4:   # (bins, tile, waves, order) pairs
             self.order_calls.append((item["id"], tuple(cands)))
             ms = list(self.order_ms[item["id"]])
         else:
@@ -82,3 +88,23 @@ def test_order_choice_over_class_instances():
     ctx = _FakeCtx(layout_ms, {})
     cache = at.tune_classes(ctx, 0, _items(3), candidates=cands)
     assert not ctx.order_calls and cache[at.class_key(0, 512, 512, 16)]["order"] == 0
+
+
+def test_split_choice():
+    at = importlib.import_module("4dzb_amd.autotune")
+    cands = [(16, 8, 4), (8, 8, 4)]
+    layout_ms = {(16, 8, 4): 2.0, (8, 8, 4): 1.5}
+    for two, want in ((0.96, 2), (0.98, 0)):   # split 2 vs the automatic rule (0) at 1.0 ms
+        split_ms = {0: 1.0, 1: 1.3, 2: two, 4: 1.01, 8: 1.2, 16: -1.0}
+        ctx = _FakeCtx(layout_ms, {0: (1.0, 2.0)}, split_ms)
+        e = at.tune_one(ctx, _items(1)[0], cands, order_items=_items(1), splits=at.SPLITS)
+        assert e["order"] == at.ORDER_LIST and e["split"] == want
+        assert e["ms"] == (two if want else 1.0)   # a kept split's time; else the order pair's
+        # the splits are timed at the chosen layout and order, the automatic rule first
+        (call,) = ctx.split_calls
+        assert call[0] == (8, 8, 4, at.ORDER_LIST, 0) and [c[4] for c in call[1:]] == list(at.SPLITS)
+        assert e["split_ms"]["16"] is None   # not exact: never chosen
+    cache = {at.class_key(0, 64, 64, 8): e}
+    assert at.split(cache, 0, 64, 64, 8) == 0 and at.split(cache, 0, 32, 32, 8) == 0
+    cache[at.class_key(0, 64, 64, 8)] = dict(e, split=4)
+    assert at.split(cache, 0, 64, 64, 8) == 4 and at.server_table(cache)[0]["split"] == 4

@@ -57,3 +57,15 @@ def test_fd_namespace_mirror_links(fd):
     out = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True).stdout
     for n in ["fd_init", "fd_launch", "fd_set_area_light", "fd_add_model"]:
         assert n in out
+
+
+def test_drivers_compile_and_bench_refuses_several_gpus():
+    """bench.py and __graft_entry__.py are valid Python; bench.py --gpus 2
+    exits with a message before it touches a device (one GPU per run)."""
+    import py_compile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for f in ("bench.py", "__graft_entry__.py"):
+        py_compile.compile(os.path.join(root, f), doraise=True)
+    r = subprocess.run(["python3", os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300, cwd=root)
+    assert r.returncode != 0 and "--gpus 1" in r.stderr

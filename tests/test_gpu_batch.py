@@ -106,6 +106,63 @@ def test_occlusion_waves_give_identical_bits(fd, wl):
     ctx.close()
 
 
+def test_list_split_gives_identical_bits(fd, wl):
+    """Any number of stretches per bin list (fd_set_list_split 1..64, or the
+    automatic rule): the same bits, alone, per instance in one batch, and per
+    class in a server."""
+    scene, ctx, items, keep = _setup(fd, wl)
+    alone = [_alone(fd, wl, scene, it) for it in items]
+    for k in (2, 5):   # 256x256 / 32x32 light, 64x200 / 64x64 light on 8x8 bins
+        it = items[k]
+        for segs in (1, 3, 16, 64):
+            print(f"[split] instance {k} alone, {segs} stretches", flush=True)
+            fr = wl.Frame("s", scene, it["rx"], it["light"], eps=1e-3, backface_cull=1,
+                          grid_log2=(it["bins"].bit_length() - 1) if it["bins"] else 4, list_split=segs)
+            run = fd.FrameRunner(fr)
+            rc, st = run.run()
+            assert rc == 0
+            sh, mk = run.outputs()
+            run.close()
+            (sh0, mk0), _ = alone[k]
+            assert np.array_equal(sh.view(np.uint32), sh0.view(np.uint32)) and np.array_equal(mk, mk0), (k, segs)
+    stream = fd.Stream()
+    splits = (1, 2, 5, 16, 64, 7, 0)
+    print(f"[split] mixed batch {splits}", flush=True)
+    for k, it in enumerate(items):
+        it["split"] = splits[k]
+        it["shadow"].upload(np.full(it["w"] * it["h"], -7.0, np.float32))
+        it["masks"].upload(np.full(it["masks"].nbytes // 4, 0xA5A5A5A5, np.uint32))
+    b = fd.Batch(ctx, items)
+    b.launch(stream.handle)
+    stream.sync()
+    for k, it in enumerate(items):
+        (sh0, mk0), st0 = alone[k]
+        rc, st = b.stats(k)
+        assert rc == 0 and st.exit_code == st0.exit_code, k
+        assert np.array_equal(it["shadow"].download(np.float32).view(np.uint32), sh0.view(np.uint32)), k
+        assert np.array_equal(it["masks"].download(np.uint32), mk0), k
+    b.close()
+    stream.close()
+    ctx.close()
+    # per class in a server (bins as each item asks; 16 where it leaves it to the default)
+    print("[split] server with per-class splits", flush=True)
+    tune = [dict(scene=0, w=it["w"], h=it["h"], res=it["light"].res_w, bins=it["bins"] or 16, tile=8,
+                 split=(4, 1, 9, 2, 32, 3, 0)[k], ms=1.0) for k, it in enumerate(items)]
+    srv = fd.Server(budget_bytes=1 << 30, streams=2, max_batch=3, eps=1e-3, tune=tune)
+    srv.add_scene(keep[0], scene.nverts, 16, keep[1], scene.ntris)
+    for k, it in enumerate(items):
+        it["masks"].upload(np.full(it["masks"].nbytes, 0x5A, np.uint8))
+        srv.submit(0, it["pos"], it["nrm"], it["w"], it["h"], it["light"], it["shadow"], it["masks"], tag=k)
+    srv.drain(120000)
+    done = srv.poll(len(items) + 2)
+    srv.close()
+    assert len(done) == len(items) and all(c.status == 0 for c in done)
+    for k, it in enumerate(items):
+        (sh0, mk0), _ = alone[k]
+        assert np.array_equal(it["shadow"].download(np.float32).view(np.uint32), sh0.view(np.uint32)), k
+        assert np.array_equal(it["masks"].download(np.uint32), mk0), k
+
+
 def test_batch_instance_against_oracle(fd, wl, oracle):
     scene, ctx, items, keep = _setup(fd, wl)
     b = fd.Batch(ctx, items)
@@ -140,13 +197,15 @@ def test_tile_widths_and_autotune(fd, wl, oracle):
     frame.tile_width = 8
     fr = fd.FrameRunner(frame)
     item = dict(pos=fr.pos, nrm=fr.nrm, w=frame.rx.w, h=frame.rx.h, light=frame.light, shadow=fr.shadow)
-    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64), (16, 8, 2), (32, 16, 2), (16, 8, 0, 1), (16, 8, 4, 2)]
+    cands = [(8, 8), (16, 8), (16, 16), (32, 32), (16, 64), (16, 8, 2), (32, 16, 2), (16, 8, 0, 1), (16, 8, 4, 2),
+             (16, 8, 0, 0, 1), (16, 8, 0, 0, 7)]
     best, ms = fr.ctx.autotune(item, cands, reps=2)
     assert 0 <= best < len(cands) and ms[best] > 0
     assert ms[1] > 0   # the reference layout itself is always exact
     assert all(m in (-1, -2) or m > 0 for m in ms)
     assert ms[best] == min(m for m in ms if m > 0)
     assert ms[7] > 0 and ms[8] > 0   # the walk orders change no bit
+    assert ms[9] > 0 and ms[10] > 0   # nor does the list split
     at = importlib.import_module("4dzb_amd.autotune")
     e = at.tune_one(fr.ctx, item, [(16, 8, 4), (8, 8, 4)], reps=2, order_items=[item, item])
     assert e["order"] in (1, 2) and e["ms"] > 0

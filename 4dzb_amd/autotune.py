@@ -6,12 +6,11 @@ side x receiver tile width x waves per occlusion work item) through the
 library's fd_autotune, which times each on the device and compares every
 output bit with the default layout (16x16 bins, 8x8 tiles, the library's
 default waves).  The fastest bit-exact layout wins.  Optionally (orders=True)
-the two walk orders of the bins' triangle lists -- the reference's, and
-largest projected triangles first -- are then timed at that layout on up to
-ORDER_SAMPLES instances of the class, and the area order is kept when it is at
-least 3% faster over them (the automatic choice per frame keeps the
-reference's order for connected meshes, where the area order wins or loses
-by up to 40% depending on the light pose, so one instance is not enough to
+the walk orders of the bins' triangle lists -- the reference's, largest
+projected triangles first, and the mesh order (largest first except the
+very largest, last) -- are then timed at that layout on up to ORDER_SAMPLES
+instances of the class, and the fastest in sum is kept (an order's effect
+changes with the light pose by up to 60%, so one instance is not enough to
 decide a class).  Optionally (splits=True) the number of stretches per bin
 triangle list is then timed at that layout and order -- the library's
 automatic rule against fixed counts -- and a fixed count is kept when it is
@@ -33,7 +32,8 @@ BINS = (8, 16, 32)
 TILES = (8, 16, 32)
 WAVES = (4, 8)
 CANDIDATES = [(b, t, w) for w in WAVES for b in BINS for t in TILES]
-ORDER_LIST, ORDER_AREA = 1, 2   # FD_TRI_ORDER_LIST / FD_TRI_ORDER_AREA
+ORDER_LIST, ORDER_AREA, ORDER_MESH = 1, 2, 3   # FD_TRI_ORDER_LIST / _AREA / _MESH
+ORDERS = (ORDER_LIST, ORDER_AREA, ORDER_MESH)
 ORDER_SAMPLES = 8               # instances of a class that decide its walk order
 SPLITS = (1, 2, 4, 8, 16)       # stretches per bin list tried against the automatic rule (0)
 
@@ -69,19 +69,22 @@ def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None, splits=
     # the walk order at that layout, summed over the given instances
     order, order_ms = 0, {}
     if order_items:
-        tl = ta = 0.0
+        tot = [0.0] * len(ORDERS)
         first = None
         for it in order_items:
-            _, oms = ctx.autotune(it, [(b, t, w, ORDER_LIST), (b, t, w, ORDER_AREA)], reps)
+            _, oms = ctx.autotune(it, [(b, t, w, o) for o in ORDERS], reps)
             if min(oms) < 0:   # not exact or no fit (cannot happen: the order changes no bit)
-                tl = ta = -1.0
+                tot = None
                 break
             first = first or oms
-            tl, ta = tl + oms[0], ta + oms[1]
-        order_ms = {"list": round(tl, 4), "area": round(ta, 4), "instances": len(order_items)}
-        if tl > 0 and ta > 0:
-            order = ORDER_AREA if ta < 0.97 * tl else ORDER_LIST
-            best_ms = first[order - 1]
+            tot = [x + m for x, m in zip(tot, oms)]
+        names = {ORDER_LIST: "list", ORDER_AREA: "area", ORDER_MESH: "mesh"}
+        order_ms = {names[o]: (round(x, 4) if tot else None) for o, x in zip(ORDERS, tot or [0.0] * len(ORDERS))}
+        order_ms["instances"] = len(order_items)
+        if tot:
+            k = min(range(len(ORDERS)), key=lambda i: tot[i])
+            order = ORDERS[k]
+            best_ms = first[k]
     # stretches per bin list at that layout and order
     split, split_ms = 0, {}
     if splits:
@@ -122,7 +125,7 @@ def layout(cache, scene, w, h, res):
 
 
 def order(cache, scene, w, h, res):
-    """Walk order of a class (ORDER_LIST / ORDER_AREA), 0 = automatic."""
+    """Walk order of a class (ORDER_LIST / _AREA / _MESH), 0 = automatic."""
     e = cache.get(class_key(scene, w, h, res))
     return e.get("order", 0) if e else 0
 

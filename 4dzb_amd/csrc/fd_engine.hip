@@ -407,7 +407,7 @@ int fd_set_occlusion_waves(fd_ctx* c, int waves) {
 }
 
 int fd_set_triangle_order(fd_ctx* c, int order) {
-    if (!c || (order != FD_TRI_ORDER_AUTO && order != FD_TRI_ORDER_LIST && order != FD_TRI_ORDER_AREA))
+    if (!c || order < FD_TRI_ORDER_AUTO || order > FD_TRI_ORDER_MESH)
         return FD_ERR_ARG;
     c->triOrder = order;
     return FD_OK;
@@ -819,7 +819,7 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
         FdLight L;
         light_setup(q.light, L);
         if (q.occ_waves != 0 && !fd_valid_waves(q.occ_waves)) { delete b; return FD_ERR_ARG; }
-        if (q.tri_order != FD_TRI_ORDER_AUTO && q.tri_order != FD_TRI_ORDER_LIST && q.tri_order != FD_TRI_ORDER_AREA) {
+        if (q.tri_order < FD_TRI_ORDER_AUTO || q.tri_order > FD_TRI_ORDER_MESH) {
             delete b;
             return FD_ERR_ARG;
         }
@@ -1114,7 +1114,7 @@ extern "C" int fd_debug_read(fd_ctx* c, const char* name, void* dst, size_t maxb
         get(a.sortedTri, np * sizeof(int));
     } else if (n == "pair_tri_walk") {
         const size_t np = std::min<unsigned long long>(s.pairs, a.pairCap);
-        get(s.status == FD_EXIT_OK && s.triOrder == FD_TRI_ORDER_AREA ? (const void*)a.pairKT : (const void*)a.sortedTri,
+        get(s.status == FD_EXIT_OK && s.triOrder != FD_TRI_ORDER_LIST ? (const void*)a.pairKT : (const void*)a.sortedTri,
             np * sizeof(int));
     } else if (n == "tri_class") {
         get(a.triClass, (size_t)a.ntris);

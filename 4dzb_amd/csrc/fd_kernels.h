@@ -7,6 +7,11 @@
 #define FD_RXB 1024   // receivers per block in the key / scatter passes
 #define FD_TRB 1024   // triangles per block in the count / scatter passes
 #define FD_TRI_CLASSES 16      // area classes of the area walk order
+// Automatic walk order of a connected mesh: the mesh order up to this many
+// pairs per 256 bins (the mean list length in 16x16 bins: a small triangle
+// touches about one bin at any grid), the list order above (dense scenes,
+// where a receiver fills from whichever object comes first in the list)
+#define FD_MESH_MAX_LIST16 7168
 #define FD_WALK_KEY_BITS 5     // bits of an area class or of the no-entry value FD_TRI_CLASSES
 #define FD_ORDER_CACHE 32768   // list entries whose walk keys the area-order pass keeps in LDS
 #define FD_ORDER_THREADS 1024  // threads per bin in the area-order pass
@@ -76,7 +81,7 @@ struct FdArgs {
     // occlusion kernel: FD_OCC_PER_SAMPLE (the contract) or FD_OCC_BIN_QUEUE
     int occVariant;
     // order of each bin's triangle list in the per-sample occlusion:
-    // FD_TRI_ORDER_AUTO, FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA
+    // FD_TRI_ORDER_AUTO, FD_TRI_ORDER_LIST, FD_TRI_ORDER_AREA or FD_TRI_ORDER_MESH
     int triOrder;
     // waves per occlusion work item (per-sample kernel): 2, 4 or 8
     int occWaves;

@@ -658,6 +658,20 @@ __device__ __forceinline__ int area_class(const float (&px)[3], const float (&py
     return min(max(7 - e, 0), FD_TRI_CLASSES - 1);
 }
 
+// Mesh order: the largest area class that comes last.  A class is
+// 7 - floor(log2(area in bin cells)) and a bin cell is 1/4^log2G of the
+// light plane, so 1/512 of the plane is class 16 - 2 log2G (8 at 16x16
+// bins); classes up to it go last.
+__device__ __forceinline__ int mesh_big_class(int log2G) { return 16 - 2 * log2G; }
+
+// Walk key of an area class: the class itself (area order, big < 0), or
+// for the mesh order the classes above big first (keys 0..14-big, larger
+// triangles first) and the classes up to big last (one key, 15 - big).
+__device__ __forceinline__ int walk_key(int cls, int big) {
+    if (big < 0) return cls;
+    return cls > big ? cls - big - 1 : FD_TRI_CLASSES - 1 - big;
+}
+
 __device__ __forceinline__ void tri_count_body(const FdArgs& a, const int blk) {
     // nb counters, then the staged bin statistics (float4 + float per bin);
     // 16-byte aligned behind the block's static counters, for the float4s
@@ -1104,14 +1118,16 @@ __device__ __forceinline__ void tri_order_body(const FdArgs& a, const int bin) {
     __shared__ int keyBase[K];
     __shared__ uint8_t keyCache[FD_ORDER_CACHE];
     const FdState* s = a.st;
-    if (s->status != FD_EXIT_OK || s->triOrder != FD_TRI_ORDER_AREA || bin >= s->nb) return;
+    if (s->status != FD_EXIT_OK || s->triOrder == FD_TRI_ORDER_LIST || bin >= s->nb) return;
     const int ts = a.triStart[bin], te = a.triEnd[bin];
     if (te <= ts) return;
     const int lane = lane_id(), w = threadIdx.x >> 6;
     int* out = (int*)a.pairKT;
+    const int big = s->triOrder == FD_TRI_ORDER_MESH ? mesh_big_class(s->log2G) : -1;
     auto key_of = [&](int i, int& tri) -> int {
         tri = a.sortedTri[i];
-        return ((unsigned)tri < (unsigned)a.ntris) ? (int)a.triClass[tri] : K - 1;
+        const int cls = ((unsigned)tri < (unsigned)a.ntris) ? (int)a.triClass[tri] : K - 1;
+        return walk_key(cls, big);
     };
     // lanes of the wave with the same key (key K: no entry)
     auto same_key = [](int key) -> uint64_t {
@@ -1212,13 +1228,19 @@ __device__ __forceinline__ void post_pairs_body(const FdArgs& a, const int blk) 
         overlaps += wsum[2][j];
     }
     s->validTris = nv;
-    // walk order of the bins' lists: the reference's (ascending id) when
-    // neighbouring triangles mostly overlap (a connected mesh: its objects'
-    // triangles stay together, and a receiver usually fills from one
-    // object), else largest projected triangles first (a triangle soup: big
-    // occluders fill receivers early)
+    // walk order of the bins' lists.  A triangle soup (neighbouring
+    // triangles mostly apart): largest projected triangles first, big
+    // occluders fill receivers early.  A connected mesh: the mesh order,
+    // largest first except those covering 1/512 of the light plane or more,
+    // which come last in list order (mostly triangles near the light; first,
+    // they would scatter each object's triangles, which a receiver often
+    // needs together) -- unless the lists are long (many objects overlap:
+    // a receiver fills from whichever object comes first in the list, so
+    // the list order, which keeps every object together, wins)
+    const bool mesh = 2 * overlaps >= pairsAdj;
+    const bool dense = s->pairs > (unsigned long long)FD_MESH_MAX_LIST16 * 256ull;
     s->triOrder = a.triOrder != FD_TRI_ORDER_AUTO ? a.triOrder
-                  : (2 * overlaps >= pairsAdj ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_AREA);
+                  : !mesh ? FD_TRI_ORDER_AREA : dense ? FD_TRI_ORDER_LIST : FD_TRI_ORDER_MESH;
     // valid receivers: binned (keys < nb, the scanned histogram at key nb)
     // plus keys out of range
     s->validSamples = (int)a.histR[(size_t)s->nb * a.nblkR] + s->keyOutOfRange;
@@ -1866,7 +1888,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     // the order is read once into a scalar register, so the list address
     // stays scalar
     const int walkOrder = uniform(s->triOrder);
-    const cint_p triList = (cint_p)(walkOrder == FD_TRI_ORDER_AREA ? (const int*)a.pairKT : a.sortedTri);
+    const cint_p triList = (cint_p)(walkOrder != FD_TRI_ORDER_LIST ? (const int*)a.pairKT : a.sortedTri);
     const ctri_p tris = (ctri_p)(a.tri);
     bool live = act;   // receiver in this wave and not yet fully shadowed
     // Prefilter (timed kernel): the wave reads its next 64 list entries and
@@ -1876,7 +1898,7 @@ __device__ __forceinline__ void occlude_body(const FdArgs& a, const int blkIn) {
     // triangle that fails them fails the test of every lane, which then skips
     // it: dropping it early changes nothing.  The counting kernel walks every
     // entry, as the reference does.
-    const int* listV = walkOrder == FD_TRI_ORDER_AREA ? (const int*)a.pairKT : a.sortedTri;
+    const int* listV = walkOrder != FD_TRI_ORDER_LIST ? (const int*)a.pairKT : a.sortedTri;
     float gx0 = 0.0f, gy0 = 0.0f, gx1 = 0.0f, gy1 = 0.0f, gz = 0.0f;
     if (!COUNT) {
         auto su = [](float v) { return __int_as_float(uniform(__float_as_int(v))); };   // into a scalar register

@@ -309,7 +309,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
             bins = t.bins_per_side;
             tw = t.tile_width;
             if (fd_valid_waves(t.occ_waves)) waves = t.occ_waves;
-            if (t.tri_order == FD_TRI_ORDER_LIST || t.tri_order == FD_TRI_ORDER_AREA) order = t.tri_order;
+            if (t.tri_order >= FD_TRI_ORDER_LIST && t.tri_order <= FD_TRI_ORDER_MESH) order = t.tri_order;
             if (t.list_split > 0 && t.list_split <= FD_LIST_SPLIT_MAX) split = t.list_split;
             break;
         }

@@ -43,7 +43,7 @@ struct FdState {
     unsigned long long pairs;  // total (bin, triangle) pairs
     unsigned long long rxTotal;
     int totalChunks;        // occlusion work items
-    int triOrder;           // order of each bin's list in the occlusion: 1 reference (ascending id), 2 area classes
+    int triOrder;           // order of each bin's list in the occlusion: 1 reference (ascending id), 2 area classes, 3 mesh
     // light-plane parameters (lightPlaneParamCal, Step2_3_defineFDbuffer.inl:5-111)
     float sampleMaxZ, triMaxZ, f;
     float S[2];             // RectangleFactor / f

@@ -29,7 +29,7 @@ EXIT_OK, EXIT_SAMPLE_MAXZ, EXIT_TRI_MAXZ, EXIT_NO_VALID_SAMPLE = 0, 1, 2, 3
 EXIT_EMPTY_AABB, EXIT_EMPTY_LP, EXIT_NO_VALID_TRI, EXIT_NO_PAIRS, EXIT_CAPACITY = 4, 5, 6, 7, 9
 
 OCC_PER_SAMPLE, OCC_BIN_QUEUE = 0, 1   # fd_set_occlusion_variant
-TRI_ORDER_AUTO, TRI_ORDER_LIST, TRI_ORDER_AREA = 0, 1, 2   # fd_set_triangle_order
+TRI_ORDER_AUTO, TRI_ORDER_LIST, TRI_ORDER_AREA, TRI_ORDER_MESH = 0, 1, 2, 3   # fd_set_triangle_order
 
 
 class AreaLight(ctypes.Structure):
@@ -347,8 +347,8 @@ class Context:
         _check(lib().fd_set_list_split(self.h, int(segs)), f"fd_set_list_split({segs})")
 
     def set_triangle_order(self, order):
-        """TRI_ORDER_AUTO / TRI_ORDER_LIST / TRI_ORDER_AREA: the order in which
-        the occlusion walks each bin's triangle list (changes no result)."""
+        """TRI_ORDER_AUTO / _LIST / _AREA / _MESH: the order in which the
+        occlusion walks each bin's triangle list (changes no result)."""
         _check(lib().fd_set_triangle_order(self.h, int(order)), f"fd_set_triangle_order({order})")
 
     def autotune(self, inst_item, candidates, reps=3):

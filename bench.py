@@ -173,7 +173,7 @@ def occlusion_roofline(g, occ_ms, workload, light_res):
                 valu_issue_profiled=prof.get("valu_issue") if prof else None,
                 occupancy_profiled=prof.get("occupancy") if prof else None,
                 kernel="k_occlude", kernel_ms=round(occ_ms, 4),
-                walk_order={1: "list", 2: "area classes"}.get(g["walk_order"], "n/a"),
+                walk_order={1: "list", 2: "area classes", 3: "mesh (area classes, largest last)"}.get(g["walk_order"], "n/a"),
                 work=dict(candidate_tests=int(cand), cull_passes=int(npass), row_visits=int(rows),
                           work_items=int(items), wave_tri_iterations=int(wave_it),
                           wave_row_iterations=int(wave_rows),
@@ -343,7 +343,7 @@ def gpu_batched(pkg, reps=3):
                 per_class=per_class,
                 autotune=dict(seconds=round(t_tune, 2), classes={k: dict(bins=v["bins"], tile=v["tile"],
                                                                         waves=v.get("waves"),
-                                                                        order={1: "list", 2: "area"}.get(v.get("order"), "auto"),
+                                                                        order={1: "list", 2: "area", 3: "mesh"}.get(v.get("order"), "auto"),
                                                                         ms=v["ms"], default_ms=v["default_ms"])
                                                                for k, v in sorted(cache.items())}))
 

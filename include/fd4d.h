@@ -147,9 +147,16 @@ int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
  *   FD_TRI_ORDER_AREA: by the area of the triangle projected on the light
  *     plane in 16 powers of two, largest first, ascending id within a
  *     class;
- *   FD_TRI_ORDER_AUTO (default): decided per frame on the device -- the
- *     list order when consecutive triangles mostly overlap on the light
- *     plane (a connected mesh), else the area order (a triangle soup).
+ *   FD_TRI_ORDER_MESH: the area order, except that triangles whose
+ *     projection covers 1/512 of the light plane or more come last (in
+ *     list order): on meshes those are mostly triangles close to the light,
+ *     which a receiver seldom needs, while putting them first scatters the
+ *     triangles of each object;
+ *   FD_TRI_ORDER_AUTO (default): decided per frame on the device -- when
+ *     consecutive triangles mostly overlap on the light plane (a connected
+ *     mesh) the mesh order, or the list order if the frame has more than
+ *     7168 x 256 pairs (dense scenes: lists longer than 7168 entries in
+ *     16 x 16 bins); else the area order (a triangle soup).
  * The batched form takes it per instance (fd_instance.tri_order, else the
  * context's setting), the serving form per class (fd_tune_entry.tri_order,
  * else automatic); fd_debug_read("pair_tri") always returns the reference's
@@ -157,6 +164,7 @@ int fd_set_occlusion_variant(fd_ctx* ctx, int variant);
 #define FD_TRI_ORDER_AUTO 0
 #define FD_TRI_ORDER_LIST 1
 #define FD_TRI_ORDER_AREA 2
+#define FD_TRI_ORDER_MESH 3
 int fd_set_triangle_order(fd_ctx* ctx, int order);
 
 /* No counterpart (a schedule knob of this implementation): waves per work
@@ -249,7 +257,7 @@ typedef struct fd_instance {
     unsigned long long max_pairs; /* pair arena of this instance; 0 = 16M */
     int occ_waves;              /* 0 = the context's setting (fd_set_occlusion_waves) */
     int tri_order;              /* 0 = the context's setting (fd_set_triangle_order), else
-                                   FD_TRI_ORDER_LIST or FD_TRI_ORDER_AREA */
+                                   FD_TRI_ORDER_LIST, _AREA or _MESH */
     int list_split;             /* 0 = the context's setting (fd_set_list_split) */
     int pad;
 } fd_instance;
@@ -280,7 +288,7 @@ typedef struct fd_tune_entry {
     int tile_width;
     float ms;                 /* measured frame time; < 0: not bit-exact */
     int occ_waves;            /* waves per occlusion work item: 2, 4 or 8 (0 = default) */
-    int tri_order;            /* walk order: FD_TRI_ORDER_LIST / _AREA (0 = automatic) */
+    int tri_order;            /* walk order: FD_TRI_ORDER_LIST / _AREA / _MESH (0 = automatic) */
     int list_split;           /* stretches per bin list (0 = automatic; fd_set_list_split) */
 } fd_tune_entry;
 typedef struct fd_server_config {

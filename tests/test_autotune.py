@@ -45,7 +45,7 @@ class _FakeCtx:
         if len(cands[0]) == 5:   # (bins, tile, waves, order, split)
             self.split_calls.append(tuple(cands))
             ms = [self.split_ms[c[4]] for c in cands]
-        elif len(cands[0]) == 4:   # (bins, tile, waves, order) pairs
+        elif len(cands[0]) == 4:   # (bins, tile, waves, order) per order
             self.order_calls.append((item["id"], tuple(cands)))
             ms = list(self.order_ms[item["id"]])
         else:
@@ -66,23 +66,23 @@ def test_order_choice_over_class_instances():
     at = importlib.import_module("4dzb_amd.autotune")
     cands = [(16, 8, 4), (8, 8, 4)]
     layout_ms = {(16, 8, 4): 2.0, (8, 8, 4): 1.5}
-    # area faster on the first instances, slower in sum: the class keeps the list order
-    order_ms = {0: (1.0, 0.8), 1: (1.0, 0.8), 2: (1.0, 0.8), 3: (1.0, 2.0)}
+    # (list, area, mesh) per instance: area fastest on the first instances,
+    # slower in sum; the class takes the fastest order in sum (mesh)
+    order_ms = {0: (1.0, 0.8, 0.9), 1: (1.0, 0.8, 0.9), 2: (1.0, 0.8, 0.9), 3: (1.0, 2.0, 0.95)}
     ctx = _FakeCtx(layout_ms, order_ms)
     cache = at.tune_classes(ctx, 0, _items(4), candidates=cands, orders=True)
     e = cache[at.class_key(0, 512, 512, 16)]
     assert (e["bins"], e["tile"], e["waves"]) == (8, 8, 4)
-    assert e["order_ms"] == {"list": 4.0, "area": 4.4, "instances": 4} and e["order"] == at.ORDER_LIST
-    assert e["ms"] == 1.0   # the order pair's time of the class's representative
-    # every order pair is timed at the chosen layout
-    assert all(c == ((8, 8, 4, at.ORDER_LIST), (8, 8, 4, at.ORDER_AREA)) for _, c in ctx.order_calls)
-    # area 3% or more faster in sum: area; less than 3%: list
-    for area, want in ((0.969, at.ORDER_AREA), (0.975, at.ORDER_LIST)):
-        ctx = _FakeCtx(layout_ms, {k: (1.0, area) for k in range(12)})
+    assert e["order_ms"] == {"list": 4.0, "area": 4.4, "mesh": 3.65, "instances": 4} and e["order"] == at.ORDER_MESH
+    assert e["ms"] == 0.9   # the chosen order's time on the class's representative
+    # every order triple is timed at the chosen layout
+    assert all(c == tuple((8, 8, 4, o) for o in at.ORDERS) for _, c in ctx.order_calls)
+    for times, want in (((1.0, 0.97, 0.99), at.ORDER_AREA), ((0.9, 0.97, 0.99), at.ORDER_LIST)):
+        ctx = _FakeCtx(layout_ms, {k: times for k in range(12)})
         e = at.tune_one(ctx, _items(1)[0], cands, order_items=_items(12)[:at.ORDER_SAMPLES])
         assert e["order"] == want and e["order_ms"]["instances"] == at.ORDER_SAMPLES
     # at most ORDER_SAMPLES instances per class decide; without orders=True, no order pairs run
-    ctx = _FakeCtx(layout_ms, {k: (1.0, 0.5) for k in range(12)})
+    ctx = _FakeCtx(layout_ms, {k: (1.0, 0.5, 0.7) for k in range(12)})
     at.tune_classes(ctx, 0, _items(12), candidates=cands, orders=True)
     assert sorted(i for i, _ in ctx.order_calls) == list(range(at.ORDER_SAMPLES))
     ctx = _FakeCtx(layout_ms, {})
@@ -96,7 +96,7 @@ def test_split_choice():
     layout_ms = {(16, 8, 4): 2.0, (8, 8, 4): 1.5}
     for two, want in ((0.96, 2), (0.98, 0)):   # split 2 vs the automatic rule (0) at 1.0 ms
         split_ms = {0: 1.0, 1: 1.3, 2: two, 4: 1.01, 8: 1.2, 16: -1.0}
-        ctx = _FakeCtx(layout_ms, {0: (1.0, 2.0)}, split_ms)
+        ctx = _FakeCtx(layout_ms, {0: (1.0, 2.0, 1.5)}, split_ms)
         e = at.tune_one(ctx, _items(1)[0], cands, order_items=_items(1), splits=at.SPLITS)
         assert e["order"] == at.ORDER_LIST and e["split"] == want
         assert e["ms"] == (two if want else 1.0)   # a kept split's time; else the order pair's

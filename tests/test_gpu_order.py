@@ -2,15 +2,19 @@
 
 fd_set_triangle_order picks the order in which the occlusion kernel walks
 each bin's list: the reference's (ascending triangle id), the largest
-triangles on the light plane first (16 power-of-two area classes), or per
-frame on the device (the list order for connected meshes, the area order
-for triangle soups).  A receiver's mask is the OR of its triangles' masks, so the order must not
+triangles on the light plane first (16 power-of-two area classes), the mesh
+order (the same, except that the triangles covering 1/512 of the light
+plane or more come last, in list order), or per frame on the device (the
+mesh order for connected meshes, the list order for dense ones, the area
+order for triangle soups).  A
+receiver's mask is the OR of its triangles' masks, so the order must not
 change a bit of any output; it only changes how soon receivers fill up.
 
 Checked here: the automatic choice, oracle bits under every order, the
-area-ordered lists as a stable sort of the reference's lists by the
-triangles' area classes (restated here in float32, and checked against their
-definition), and the batched form over a soup.
+area- and mesh-ordered lists as stable sorts of the reference's lists by
+the triangles' area classes (restated here in float32, and checked against
+their definition), and the batched and served forms with per-instance and
+per-class orders.
 """
 import numpy as np
 import pytest
@@ -20,7 +24,7 @@ from parity import compare
 
 pytestmark = pytest.mark.gpu
 
-LIST, AREA, AUTO = 1, 2, 0
+LIST, AREA, MESH, AUTO = 1, 2, 3, 0
 
 
 def _soup(wl, light_res=64):
@@ -48,19 +52,33 @@ def _run(fd, frame, order):
     return fr
 
 
+def _dense(wl):
+    # 1M triangles of 400-triangle spheres under a close light: about 5M
+    # pairs, lists of ~20k entries in 16x16 bins
+    scene = wl.serve_scenes(sizes=(1_000_000,), seeds=(3,))[0]
+    rx = wl.receivers_heightfield(256, 256, 8.0, 8.0)
+    return wl.Frame("dense", scene, rx, wl.LightDesc(1.0, 1.0, (-1.33, -1.05, 4.17), res_w=32, res_h=32),
+                    eps=1e-3, backface_cull=1)
+
+
 def _soup_small(wl):
     # few enough work items that the frame splits its bins' lists into
     # stretches (merged by atomic OR), walked in area order
     return wl.boxes_1080p(w=160, h=90, ntri=60_000, light_res=40)
 
 
-@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", LIST)])
+@pytest.mark.parametrize("which,expect", [("soup", AREA), ("soup32", AREA), ("soup-small", AREA), ("mesh", MESH),
+                                          ("dense", LIST)])
 def test_auto_choice_and_oracle(fd, wl, oracle, which, expect):
     frame = {"soup": lambda: _soup(wl), "soup32": lambda: _soup(wl, 32), "soup-small": lambda: _soup_small(wl),
-             "mesh": lambda: _mesh(wl)}[which]()
+             "mesh": lambda: _mesh(wl), "dense": lambda: _dense(wl)}[which]()
     print(f"[order] auto {which}", flush=True)
     fr = _run(fd, frame, AUTO)
-    assert int(fd.read_state(fr.ctx)["triOrder"]) == expect
+    st = fd.read_state(fr.ctx)
+    assert int(st["triOrder"]) == expect
+    # the rule: meshes up to 7168 pairs per 256 bins take the mesh order, denser ones the list order
+    if expect in (MESH, LIST):
+        assert (int(st["pairs"]) > 7168 * 256) == (expect == LIST)
     rep, fr, o = compare(fd, oracle, frame, fr=fr)
     bad = {k: v for k, v in rep.items() if v}
     assert not bad, bad
@@ -72,10 +90,10 @@ def test_orders_give_identical_bits(fd, wl, which):
     frame = _soup(wl) if which == "soup" else _mesh(wl)
     print(f"[order] forced orders {which}", flush=True)
     digests, pair_lists = {}, {}
-    for order in (LIST, AREA, AUTO):
+    for order in (LIST, AREA, MESH, AUTO):
         fr = _run(fd, frame, order)
         chosen = int(fd.read_state(fr.ctx)["triOrder"])
-        assert chosen == (order if order else chosen) and chosen in (LIST, AREA)
+        assert chosen == (order if order else chosen) and chosen in (LIST, AREA, MESH)
         sh, mk = fr.outputs()
         digests[order] = _digest(sh, mk)
         pair_lists[order] = fr.ctx.debug_read("pair_tri", np.int32)   # always the reference's order
@@ -84,7 +102,7 @@ def test_orders_give_identical_bits(fd, wl, which):
             assert np.array_equal(walk, pair_lists[order])
         fr.close()
     assert len(set(digests.values())) == 1, digests
-    assert np.array_equal(pair_lists[LIST], pair_lists[AREA]) and np.array_equal(pair_lists[LIST], pair_lists[AUTO])
+    assert all(np.array_equal(pair_lists[LIST], pair_lists[o]) for o in (AREA, MESH, AUTO))
 
 
 def _area_classes(tris, light, s):
@@ -111,13 +129,24 @@ def _area_classes(tris, light, s):
         return np.where(cells > 0, cls, 15), cells.astype(np.float64)
 
 
-def test_area_lists_are_stable_class_sort(fd, wl):
-    frame = _soup(wl)
-    print("[order] area lists", flush=True)
-    fr = _run(fd, frame, AREA)
+def _walk_keys(cls, order, log2G):
+    """Walk key per class: the class (area order); for the mesh order the
+    classes above big = 16 - 2 log2G (1/512 of the light plane) first, in
+    class order, and the classes up to big last, as one key."""
+    if order == AREA:
+        return cls
+    big = 16 - 2 * log2G
+    return np.where(cls > big, cls - big - 1, 15 - big)
+
+
+@pytest.mark.parametrize("order,which", [(AREA, "soup"), (MESH, "soup"), (MESH, "mesh")])
+def test_walked_lists_are_stable_class_sorts(fd, wl, order, which):
+    frame = _soup(wl) if which == "soup" else _mesh(wl)
+    print(f"[order] walked lists, order {order}, {which}", flush=True)
+    fr = _run(fd, frame, order)
     ctx = fr.ctx
     s = fd.read_state(ctx)
-    assert int(s["triOrder"]) == AREA
+    assert int(s["triOrder"]) == order
     cls = ctx.debug_read("tri_class", np.uint8)
     tris = fd.read_tris(ctx)
     light = fd.read_light(ctx)
@@ -141,15 +170,25 @@ def test_area_lists_are_stable_class_sort(fd, wl):
     hi = np.where(c == 0, np.inf, 2.0 ** (8 - c))
     ok = (cells >= lo * (1 - 1e-5)) & (cells < hi * (1 + 1e-5))
     assert ok.all(), (int((~ok).sum()), cells[~ok][:5], c[~ok][:5])
-    assert len(np.unique(c)) >= 4   # the soup spans several classes
-    # each bin's walked list: the reference's list stably sorted by class
+    assert len(np.unique(c)) >= 4   # the frame spans several classes
+    # each bin's walked list: the reference's list stably sorted by walk key
+    keys = _walk_keys(cls.astype(np.int64), order, int(s["log2G"]))
+    if order == MESH:
+        # the last key holds exactly the triangles covering 1/512 of the
+        # light plane or more (G x G bin cells; within 1e-5 of the boundary
+        # either side)
+        frac = cells / float(int(s["G"]) ** 2)
+        last = keys[used] == keys.max()
+        assert (last | (frac < (1 + 1e-5) / 512)).all() and (~last | (frac >= (1 - 1e-5) / 512)).all()
+        if which == "soup":
+            assert last.any() and not last.all()   # the soup has such triangles, and others
     nonempty = 0
     for b in range(len(ts)):
         if te[b] <= ts[b]:
             continue
         nonempty += 1
         ref = pairs[ts[b]:te[b]]
-        expw = ref[np.argsort(cls[ref], kind="stable")]
+        expw = ref[np.argsort(keys[ref], kind="stable")]
         assert np.array_equal(walk[ts[b]:te[b]], expw), b
     assert nonempty > 0
 
@@ -197,7 +236,7 @@ def test_batch_over_soup(fd, wl):
     b.close()
     # per-instance orders in one batch: the same bits
     print("[order] batch with per-instance orders", flush=True)
-    for it, order in zip(items, (LIST, AREA, LIST)):
+    for it, order in zip(items, (LIST, AREA, MESH)):
         it["order"] = order
         it["shadow"].upload(np.zeros(it["shadow"].nbytes, np.uint8))
         it["masks"].upload(np.full(it["masks"].nbytes, 0xA5, np.uint8))
@@ -212,7 +251,7 @@ def test_batch_over_soup(fd, wl):
     # per-class orders from a server's tuning table: the same bits
     print("[order] server with per-class orders", flush=True)
     tune = [dict(scene=0, w=it["w"], h=it["h"], res=it["light"].res_w, bins=16, tile=8, order=order, ms=1.0)
-            for it, order in zip(items, (AREA, LIST, AREA))]
+            for it, order in zip(items, (AREA, LIST, MESH))]
     srv = fd.Server(budget_bytes=1 << 30, streams=2, max_batch=2, eps=1e-3, tune=tune)
     srv.add_scene(verts, scene.nverts, 16, idx, scene.ntris)
     for k, it in enumerate(items):

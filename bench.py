@@ -403,14 +403,18 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
         svc.append((time.perf_counter() - a) * 1e3)
         assert rc == 0
         ref.append(_digest(outs[k]))
-    # autotuner: one representative request per class (scene, size, grid)
+    # autotuner: per class (scene, size, grid) the layout on its first
+    # request, the walk order over up to 8 of its requests, the list split
     at = pkg.autotune
     t_tune = time.perf_counter()
     cache = {}
+    per_scene = {}
     for k, q in enumerate(reqs):
         r, pos, nrm = rx[(q["w"], q["h"])]
-        item = dict(pos=pos, nrm=nrm, w=q["w"], h=q["h"], light=lights[k], shadow=outs[k][0], max_pairs=64 << 20)
-        at.tune_classes(ctxs[q["scene"]], q["scene"], [item], cache, reps=2, splits=True)
+        per_scene.setdefault(q["scene"], []).append(
+            dict(pos=pos, nrm=nrm, w=q["w"], h=q["h"], light=lights[k], shadow=outs[k][0], max_pairs=64 << 20))
+    for sc, items in sorted(per_scene.items()):
+        at.tune_classes(ctxs[sc], sc, items, cache, reps=2, orders=True, splits=True)
     t_tune = time.perf_counter() - t_tune
     for c in ctxs:
         c.close()
@@ -487,7 +491,9 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
                               non_default=sum(1 for k, v in cache.items()
                                               if (v["bins"], v["tile"], v.get("waves"))
                                               != pkg.autotune.default_layout(int(k.split(":")[2]))),
-                              split_classes=sum(1 for v in cache.values() if v.get("split"))))
+                              split_classes=sum(1 for v in cache.values() if v.get("split")),
+                              orders={o: sum(1 for v in cache.values() if v.get("order") == i)
+                                      for i, o in ((1, "list"), (2, "area"), (3, "mesh"))}))
 
 
 def gpu_target_config(pkg, steps=10):

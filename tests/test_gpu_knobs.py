@@ -56,3 +56,55 @@ def test_knob_combination_matches_oracle(fd, wl, oracle, k, large):
     assert not bad, (k, bad)
     if order:
         assert int(st["triOrder"]) == order or o.exit != 0
+
+
+def test_batch_of_random_knobs_matches_oracle(fd, wl, oracle):
+    """One scene, 12 instances with their own receivers, light, bin grid,
+    tile width, waves, walk order and list split in one batch; the batch is
+    captured as a HIP graph and replayed twice (outputs poisoned between the
+    replays); every replay's outputs equal the oracle's, instance by
+    instance."""
+    r = np.random.default_rng(77)
+    scene = wl.tris_clusters(15000, 5, box=(-4, 4, -4, 4, 0.5, 3.5))
+    ctx = fd.Context()
+    ctx.set_backface_cull(1)
+    ctx.set_scenes_esp(1e-3)
+    verts, idx = fd.DeviceBuffer.from_array(scene.verts), fd.DeviceBuffer.from_array(scene.idx)
+    ctx.add_model(verts, scene.nverts, 16, idx, scene.ntris)
+    items, refs = [], []
+    for k in range(12):
+        w, h = int(r.integers(32, 300)), int(r.integers(32, 300))
+        rx = wl.receivers_heightfield(w, h, 4.0, 4.0 * h / w)
+        res = int(r.choice([1, 7, 16, 32, 33, 64]))
+        light = wl.LightDesc(float(r.uniform(0.3, 1.0)), float(r.uniform(0.3, 1.0)),
+                             (float(r.uniform(-1, 1)), float(r.uniform(-1, 1)), float(r.uniform(4, 7))), res_w=res, res_h=res)
+        bins = int(r.choice([8, 16, 32, 64]))
+        it = dict(w=w, h=h, light=light, bins=bins, tile=int(r.choice([8, 16, 32, 64])),
+                  waves=int(r.choice([0, 2, 4, 8])), order=int(r.choice([0, 1, 2, 3])),
+                  split=int(r.choice([0, 1, 3, 16])), pos=fd.DeviceBuffer.from_array(rx.pos),
+                  nrm=fd.DeviceBuffer.from_array(rx.nrm), shadow=fd.DeviceBuffer(w * h * 4),
+                  masks=fd.DeviceBuffer(w * h * fd.mask_words(res) * 4))
+        items.append(it)
+        o = oracle.run(wl.Frame(f"b{k}", scene, rx, light, eps=1e-3, backface_cull=1,
+                                grid_log2=bins.bit_length() - 1))
+        refs.append(o)
+    stream = fd.Stream()
+    b = fd.Batch(ctx, items)
+    b.capture(stream.handle)
+    for rep in range(2):
+        print(f"[knobs] batch replay {rep}", flush=True)
+        for it in items:
+            it["shadow"].upload(np.full(it["w"] * it["h"], -3.0, np.float32))
+            it["masks"].upload(np.full(it["masks"].nbytes // 4, 0x5A5A5A5A, np.uint32))
+        b.launch(stream.handle)
+        stream.sync()
+        for k, (it, o) in enumerate(zip(items, refs)):
+            rc, st = b.stats(k)
+            assert rc == 0 and st.exit_code == o.exit, (rep, k)
+            sh = it["shadow"].download(np.float32)
+            mk = it["masks"].download(np.uint32)
+            assert np.array_equal(sh.view(np.uint32), np.asarray(o.shadow, np.float32).view(np.uint32)), (rep, k)
+            assert np.array_equal(mk, o.masks.reshape(mk.shape)), (rep, k)
+    b.close()
+    stream.close()
+    ctx.close()

@@ -108,3 +108,46 @@ def test_batch_of_random_knobs_matches_oracle(fd, wl, oracle):
     b.close()
     stream.close()
     ctx.close()
+
+
+def test_server_with_random_tuning_matches_oracle(fd, wl, oracle):
+    """Two resident scenes, 10 requests of distinct classes, a tuning table
+    with a random layout, waves, walk order and list split per class; every
+    request's outputs equal the oracle's on the tuned bin grid."""
+    r = np.random.default_rng(91)
+    scenes = [wl.tris_clusters(12000, 8, box=(-4, 4, -4, 4, 0.5, 3.5)), wl.spheres(20, 9, box=(-4, 4, -4, 4, 0.5, 3))]
+    dev = [(fd.DeviceBuffer.from_array(sc.verts), fd.DeviceBuffer.from_array(sc.idx)) for sc in scenes]
+    reqs, tune = [], []
+    for k in range(10):
+        sc = k % 2
+        w, h = 40 + 17 * k, 64 + 9 * k   # distinct sizes: one class per request
+        res = int(r.choice([4, 16, 32, 40, 64]))
+        bins = int(r.choice([8, 16, 32]))
+        rx = wl.receivers_heightfield(w, h, 4.0, 4.0 * h / w)
+        light = wl.LightDesc(float(r.uniform(0.3, 1.0)), float(r.uniform(0.3, 1.0)),
+                             (float(r.uniform(-1, 1)), float(r.uniform(-1, 1)), float(r.uniform(4, 7))), res_w=res, res_h=res)
+        tune.append(dict(scene=sc, w=w, h=h, res=res, bins=bins, tile=int(r.choice([8, 16, 32])),
+                         waves=int(r.choice([2, 4, 8])), order=int(r.choice([0, 1, 2, 3])),
+                         split=int(r.choice([0, 1, 4, 16])), ms=1.0))
+        o = oracle.run(wl.Frame(f"s{k}", scenes[sc], rx, light, eps=1e-3, backface_cull=1,
+                                grid_log2=bins.bit_length() - 1))
+        reqs.append(dict(scene=sc, w=w, h=h, light=light, o=o, pos=fd.DeviceBuffer.from_array(rx.pos),
+                         nrm=fd.DeviceBuffer.from_array(rx.nrm), shadow=fd.DeviceBuffer(w * h * 4),
+                         masks=fd.DeviceBuffer(w * h * fd.mask_words(res) * 4)))
+    srv = fd.Server(budget_bytes=1 << 30, streams=2, max_batch=3, eps=1e-3, tune=tune)
+    for (v, i), sc in zip(dev, scenes):
+        srv.add_scene(v, sc.nverts, 16, i, sc.ntris)
+    for k, q in enumerate(reqs):
+        q["masks"].upload(np.full(q["masks"].nbytes, 0xA5, np.uint8))
+        srv.submit(q["scene"], q["pos"], q["nrm"], q["w"], q["h"], q["light"], q["shadow"], q["masks"], tag=k)
+    print("[knobs] server drain", flush=True)
+    srv.drain(120000)
+    done = sorted(srv.poll(len(reqs) + 2), key=lambda c: c.user_tag)
+    srv.close()
+    assert len(done) == len(reqs) and all(c.status == 0 for c in done)
+    for k, q in enumerate(reqs):
+        o = q["o"]
+        sh = q["shadow"].download(np.float32)
+        mk = q["masks"].download(np.uint32)
+        assert np.array_equal(sh.view(np.uint32), np.asarray(o.shadow, np.float32).view(np.uint32)), k
+        assert np.array_equal(mk, o.masks.reshape(mk.shape)), k

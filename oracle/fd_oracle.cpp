@@ -25,6 +25,12 @@
 //   * min/max ignore NaN (fminf/fmaxf) and order -0 below +0
 //
 // Parity status per function: see the "pins" list in tests/test_oracle.py.
+// One decision is parity unpinned: the bin-queue (CTA) variant's
+// group-level triangle-plane decision (occludeCTA below) has no check
+// beyond the GPU kernel agreeing with this restatement -- no ray-cast
+// geometry in which it changes a bit exists; its row primitives and masks
+// are pinned (tests/test_oracle_cta.py).  That variant is outside the
+// bit-exact contract of the per-sample path.
 #include <algorithm>
 #include <atomic>
 #include <chrono>

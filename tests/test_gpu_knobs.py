@@ -1,5 +1,5 @@
-"""Schedule knobs together (MI355X only): seeded random small frames, each
-with its own bin grid, receiver tile width, waves per occlusion work item,
+"""Schedule knobs together (MI355X only): seeded random small frames (a third
+of them under a tilted light), each with its own bin grid, receiver tile width, waves per occlusion work item,
 walk order of the bins' lists and list split, all compared bit for bit with
 the oracle (which runs the same bin grid; the other knobs are not part of
 its model).  No knob may change a stage output the oracle restates, a mask
@@ -34,6 +34,9 @@ def _case(wl, k, large=False):
     res = int(r.choice([1, 5, 8, 16, 31, 32, 33, 40, 64]))
     pos = (float(r.uniform(-1.5, 1.5)), float(r.uniform(-1.5, 1.5)), float(r.uniform(4.0, 7.0)))
     light = wl.LightDesc(float(r.uniform(0.3, 1.2)), float(r.uniform(0.3, 1.2)), pos, res_w=res, res_h=res)
+    if k % 3 == 2:   # a tilted light: the non-axial plane coefficients
+        light.view_dir = (float(r.uniform(-0.35, 0.35)), float(r.uniform(-0.35, 0.35)), -1.0)
+        light.up_dir = (float(r.uniform(-0.3, 0.3)), 1.0, float(r.uniform(-0.2, 0.2)))
     fr = wl.Frame(f"knobs-{k}", sc, rx, light, eps=1e-3, backface_cull=int(r.random() < 0.8),
                   grid_log2=int(r.integers(3, 7)), tile_width=int(r.choice([8, 16, 32, 64])),
                   occ_waves=int(r.choice([0, 2, 4, 8])), list_split=int(r.choice([0, 0, 1, 2, 5, 16, 64])))

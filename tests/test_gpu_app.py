@@ -35,6 +35,28 @@ def test_gbuffer_bit_exact(fd, wl, w, h, nbox):
         assert np.array_equal(got.view(np.uint32), ref[k].view(np.uint32)), f"target g{k}"
 
 
+@pytest.mark.parametrize("k", range(6))
+def test_gbuffer_random_cameras_bit_exact(fd, wl, k):
+    """Seeded random boxes scenes and cameras (eye, target, up, field of
+    view, ground size): all four targets bit-identical to the host ray cast."""
+    r = np.random.default_rng(300 + k)
+    w, h, nbox = int(r.integers(32, 400)), int(r.integers(32, 300)), int(r.integers(0, 31))
+    sc = wl.gbuffer_scene_boxes(w, h, nbox, int(r.integers(1, 1000)))
+    ang = float(r.uniform(0, 2 * np.pi))
+    dist, height = float(r.uniform(8, 18)), float(r.uniform(3, 14))
+    sc.eye = (dist * np.cos(ang), dist * np.sin(ang), height)
+    sc.at = (float(r.uniform(-2, 2)), float(r.uniform(-2, 2)), float(r.uniform(0, 1)))
+    sc.up = (float(r.uniform(-0.2, 0.2)), float(r.uniform(-0.2, 0.2)), 1.0)
+    sc.fovy_deg = float(r.uniform(30, 80))
+    sc.ground_half = float(r.uniform(6, 14))
+    print(f"[gbuffer] random {k}: {w}x{h} boxes {nbox} eye {sc.eye}", flush=True)
+    ref = wl.gbuffer_boxes_host(sc)
+    g = _produce(fd, sc)
+    for t in range(4):
+        got = g[t].download(np.float32).reshape(h, w, 4)
+        assert np.array_equal(got.view(np.uint32), ref[t].view(np.uint32)), (k, f"target g{t}")
+
+
 def test_gbuffer_position_normal_only(fd, wl):
     sc = wl.gbuffer_scene_boxes(200, 120, 20, 3)
     g = _produce(fd, sc, targets=2, materials=False)

@@ -125,3 +125,35 @@ def test_pipeline_gbuffer_visibility_shade(fd, wl, oracle):
         assert ok.any(axis=1).mean() > 0.3
     strm.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_shade_random_inputs(fd, wl, oracle, k):
+    """Seeded random G-buffer scene, shadow values, 1-8 light samples, camera
+    and light colours: the shading pass against the oracle's restatement of
+    phoneShading.shf, within the same tolerance."""
+    r = np.random.default_rng(500 + k)
+    w, h = int(r.integers(40, 300)), int(r.integers(40, 200))
+    sc = wl.gbuffer_scene_boxes(w, h, int(r.integers(0, 25)), int(r.integers(1, 1000)))
+    host = wl.gbuffer_boxes_host(sc)
+    g = _produce(fd, sc)
+    sh = r.uniform(0, 1, w * h).astype(np.float32)
+    sh[r.random(w * h) < 0.1] = 0.0
+    sh[r.random(w * h) < 0.1] = 1.0
+    shadow = fd.DeviceBuffer.from_array(sh)
+    samples = [np.array([r.uniform(-5, 5), r.uniform(-5, 5), r.uniform(4, 10)]) for _ in range(int(r.integers(1, 9)))]
+    cam = (float(r.uniform(-10, 10)), float(r.uniform(-15, -5)), float(r.uniform(4, 12)))
+    La, Ld, Ls = (tuple(float(x) for x in r.uniform(0, 1, 3)) for _ in range(3))
+    rgba = fd.DeviceBuffer(w * h * 16)
+    strm = fd.Stream()
+    print(f"[shade] random {k}: {w}x{h}, {len(samples)} samples", flush=True)
+    fd.shade(w, h, samples, cam, La, Ld, Ls, shadow, g[0], g[1], g[2], g[3], rgba, stream=strm)
+    strm.sync()
+    strm.close()
+    got = rgba.download(np.float32).reshape(-1, 4)
+    want = oracle.shade(sh, host[0].reshape(-1, 4), host[1].reshape(-1, 4), host[2].reshape(-1, 4),
+                        host[3].reshape(-1, 4), samples, cam, La, Ld, Ls)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert ok.any()
+    assert np.allclose(got[ok], want[ok], rtol=SHADE_RTOL, atol=SHADE_ATOL)

@@ -111,3 +111,40 @@ def test_server_scene_moves_are_ordered(fd, wl):
         assert np.array_equal(sh.download(np.float32).view(np.uint32), exp[step][0].view(np.uint32)), (k, step)
         assert np.array_equal(mk.download(np.uint32), exp[step][1]), (k, step)
     srv.close()
+
+
+def test_server_tight_budget(fd, wl, oracle):
+    """A 64 MiB workspace pool: a request that can never fit is refused at
+    submission (FD_ERR_CAPACITY) and the others wait for workspace to free
+    up; every admitted request completes with the oracle's bits."""
+    scene = wl.tris_clusters(8000, 21, box=(-6, 6, -6, 6, 0.5, 4))
+    srv = fd.Server(budget_bytes=64 << 20, streams=2, max_batch=4, eps=1e-3, max_pairs=1 << 20)
+    v, i = fd.DeviceBuffer.from_array(scene.verts), fd.DeviceBuffer.from_array(scene.idx)
+    srv.add_scene(v, scene.nverts, 16, i, scene.ntris)
+    # 2M receivers need more workspace than the whole pool
+    big = wl.receivers_heightfield(1920, 1080, 8.0, 4.5)
+    bp, bn = fd.DeviceBuffer.from_array(big.pos), fd.DeviceBuffer.from_array(big.nrm)
+    bs = fd.DeviceBuffer(1920 * 1080 * 4)
+    with pytest.raises(fd.FdError, match=str(fd.FD_ERR_CAPACITY)):
+        srv.submit(0, bp, bn, 1920, 1080, wl.LightDesc(1.0, 1.0, (0, 0, 6), res_w=16, res_h=16), bs, None, tag=99)
+    jobs = []
+    for k in range(8):
+        w, h = 160 + 24 * k, 256 - 12 * k
+        R = (8, 16, 32, 33)[k % 4]
+        rx = wl.receivers_heightfield(w, h, 8.0, 8.0 * h / w)
+        light = wl.LightDesc(1.0, 1.0, (0.2 * k - 0.7, 0.3, 5.0 + 0.2 * k), res_w=R, res_h=R)
+        jobs.append(dict(rx=rx, light=light, pos=fd.DeviceBuffer.from_array(rx.pos), nrm=fd.DeviceBuffer.from_array(rx.nrm),
+                         shadow=fd.DeviceBuffer(w * h * 4), masks=fd.DeviceBuffer(w * h * fd.mask_words(R) * 4),
+                         o=oracle.run(wl.Frame(f"t{k}", scene, rx, light, eps=1e-3, backface_cull=1))))
+    print("[server] tight budget: 8 requests", flush=True)
+    for k, j in enumerate(jobs):
+        srv.submit(0, j["pos"], j["nrm"], j["rx"].w, j["rx"].h, j["light"], j["shadow"], j["masks"], tag=k)
+    srv.drain(60000)
+    done = sorted(srv.poll(16), key=lambda c: c.user_tag)
+    srv.close()
+    assert [c.user_tag for c in done] == list(range(8)) and all(c.status == 0 for c in done)
+    for k, j in enumerate(jobs):
+        o = j["o"]
+        assert np.array_equal(j["shadow"].download(np.float32).view(np.uint32),
+                              np.asarray(o.shadow, np.float32).view(np.uint32)), k
+        assert np.array_equal(j["masks"].download(np.uint32), o.masks.ravel()), k

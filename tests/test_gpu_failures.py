@@ -196,3 +196,25 @@ def test_counting_kernel(fd, oracle, wl, light_res):
     rc, _ = run.run()
     assert rc == 0
     run.close()
+
+
+def test_knob_arguments_are_checked(fd, wl):
+    """Out-of-range schedule knobs are refused with FD_ERR_ARG (context
+    setters, batch instances, the autotuner's candidates) and leave the
+    context usable."""
+    frame = wl.plane_256(ntri=500, res=64, light_res=8)
+    run = fd.FrameRunner(frame)
+    lib, h = fd.lib(), run.ctx.h
+    for f, bad in ((lib.fd_set_list_split, (-1, 65)), (lib.fd_set_triangle_order, (-1, 4)),
+                   (lib.fd_set_occlusion_waves, (1, 3, 16))):
+        for v in bad:
+            assert f(h, v) == fd.FD_ERR_ARG, (f, v)
+    item = dict(pos=run.pos, nrm=run.nrm, w=64, h=64, light=frame.light, shadow=run.shadow, masks=run.masks)
+    for key, v in (("split", 65), ("order", 4), ("waves", 3)):
+        with pytest.raises(fd.FdError, match="fd_batch_create"):
+            fd.Batch(run.ctx, [dict(item, **{key: v})])
+    with pytest.raises(fd.FdError, match="fd_autotune"):
+        run.ctx.autotune(item, [(16, 8, 0, 0, 65)], reps=1)
+    rc, st = run.run()
+    assert rc == 0 and st.exit_code == 0
+    run.close()

@@ -38,14 +38,27 @@ ORDER_SAMPLES = 8               # instances of a class that decide its walk orde
 SPLITS = (1, 2, 4, 8, 16)       # stretches per bin list tried against the automatic rule (0)
 
 
-def default_waves(res):
+WAVES8_ITEMS = 8192   # FD_WAVES8_ITEMS: frames of up to this many work items default to 8 waves
+
+
+def work_items(w, h, bins=16):
+    """Occlusion work items of a w x h frame (64 receivers of a bin each, at
+    most: the library's chunk slots)."""
+    return (w * h + 63) // 64 + bins * bins + 1
+
+
+def default_waves(res, w=None, h=None, bins=16):
     """The library's default waves per occlusion work item for a res x res
-    light grid (fd_set_occlusion_waves)."""
-    return 8 if res > 32 else 4
+    light grid (fd_set_occlusion_waves): 8 for 64-bit rows and for frames of
+    up to WAVES8_ITEMS work items, else 4 (a frame of unknown size counts
+    as large)."""
+    if res > 32:
+        return 8
+    return 8 if w is not None and h is not None and work_items(w, h, bins) <= WAVES8_ITEMS else 4
 
 
-def default_layout(res):
-    return (16, 8, default_waves(res))
+def default_layout(res, w=None, h=None):
+    return (16, 8, default_waves(res, w, h))
 
 
 def class_key(scene, w, h, res):
@@ -62,7 +75,7 @@ def tune_one(ctx, item, candidates=CANDIDATES, reps=3, order_items=None, splits=
     entry."""
     t0 = time.perf_counter()
     best, ms = ctx.autotune(item, candidates, reps)
-    dl = default_layout(item["light"].res_w)
+    dl = default_layout(item["light"].res_w, item["w"], item["h"])
     default = ms[candidates.index(dl)] if dl in candidates else None
     b, t, w = candidates[best][:3] if best >= 0 else dl   # nothing bit-exact besides the reference: the default
     best_ms = ms[best] if best >= 0 else default
@@ -121,7 +134,7 @@ def tune_classes(ctx, scene, items, cache=None, candidates=CANDIDATES, reps=3, o
 def layout(cache, scene, w, h, res):
     """(bins, tile, waves) of a class, or the default layout."""
     e = cache.get(class_key(scene, w, h, res))
-    return (e["bins"], e["tile"], e.get("waves", default_waves(res))) if e else default_layout(res)
+    return (e["bins"], e["tile"], e.get("waves", default_waves(res, w, h, e["bins"]))) if e else default_layout(res, w, h)
 
 
 def order(cache, scene, w, h, res):
@@ -143,7 +156,7 @@ def server_table(cache):
         scene, wh, res = key.split(":")
         w, h = wh.split("x")
         out.append(dict(scene=int(scene), w=int(w), h=int(h), res=int(res), bins=e["bins"], tile=e["tile"],
-                        waves=e.get("waves", default_waves(int(res))), order=e.get("order", 0),
+                        waves=e.get("waves", default_waves(int(res), int(w), int(h), e["bins"])), order=e.get("order", 0),
                         split=e.get("split", 0), ms=e["ms"] or 0.0))
     return out
 

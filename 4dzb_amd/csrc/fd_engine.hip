@@ -92,7 +92,7 @@ struct fd_ctx {
     int tileLog2 = 3;     // 8x8 receiver tiles
     int occVariant = FD_OCC_PER_SAMPLE;
     int triOrder = FD_TRI_ORDER_AUTO;
-    int occWaves = 0;     // 0: fd_default_waves of the frame's row width
+    int occWaves = 0;     // 0: fd_default_waves of the frame's row width and work items
     int listSplit = 0;    // 0: the automatic rule
     size_t maxTris = 0, maxVerts = 0;
     int W = 0, H = 0;
@@ -239,7 +239,7 @@ int frame_args(fd_ctx* c, FdArgs& a) {
               c->masks, c->pairCap);
     a.occVariant = c->occVariant;
     a.triOrder = c->triOrder;
-    a.occWaves = c->occWaves ? c->occWaves : fd_default_waves(a.L.wpr);
+    a.occWaves = c->occWaves ? c->occWaves : fd_default_waves(a.L.wpr, a.chunkSlots);
     apply_split(a, c->listSplit);
     return FD_OK;
 }
@@ -828,7 +828,8 @@ extern "C" int fd_batch_create(fd_ctx* c, const fd_instance* in, int n, fd_batch
         fill_args(b->args[i], md, (const float4*)q.d_position4, (const float4*)q.d_normal4, q.width, q.height, L,
                   c->eps, c->cull, g, tl, q.d_shadow, q.d_masks, cap);
         b->args[i].triOrder = q.tri_order ? q.tri_order : c->triOrder;
-        b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves ? c->occWaves : fd_default_waves(L.wpr);
+        b->args[i].occWaves = q.occ_waves ? q.occ_waves : c->occWaves ? c->occWaves
+                                                     : fd_default_waves(L.wpr, b->args[i].chunkSlots);
         apply_split(b->args[i], q.list_split ? q.list_split : c->listSplit);   // after occWaves
         FdArgs probe = b->args[i];
         off[i + 1] = off[i] + ((layout(probe, nullptr, false) + 255) & ~(size_t)255);

@@ -101,10 +101,12 @@ struct FdArgs {
 
 // Waves per occlusion work item (per-sample kernel) unless a frame asks for
 // another count (2, 4 or 8): the waves share a work item's receivers, LDS
-// rows and full-row masks and split its triangle list.  4 for 32-bit rows
-// (8 KiB of rows per work item), 8 for 64-bit rows (32 KiB: LDS holds 4
-// work items per CU, so 8 waves each fill it).
-inline int fd_default_waves(int wpr) { return wpr == 2 ? 8 : 4; }
+// rows and full-row masks and split its triangle list.  8 for 64-bit rows
+// (32 KiB of rows per work item: LDS holds 4 work items per CU, so 8 waves
+// each fill it) and for frames of up to FD_WAVES8_ITEMS work items (too few
+// to fill the GPU with 4 waves each); 4 for larger frames of 32-bit rows.
+#define FD_WAVES8_ITEMS 8192
+inline int fd_default_waves(int wpr, long items) { return (wpr == 2 || items <= FD_WAVES8_ITEMS) ? 8 : 4; }
 inline bool fd_valid_waves(int w) { return w == 2 || w == 4 || w == 8; }
 
 // Occlusion classes of a batched launch: 32-bit / 64-bit rows x 2 / 4 / 8

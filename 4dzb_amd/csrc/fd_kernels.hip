@@ -2621,7 +2621,7 @@ hipError_t enqueue_frame(const FdArgs& a, hipStream_t st, hipEvent_t* ev) {
             if (wpr == 1) k_occlude<uint32_t, true, 1><<<chunks, 64, shm, st>>>(ao);
             else k_occlude<uint64_t, true, 1><<<chunks, 64, shm, st>>>(ao);
         } else {
-            const int nw = fd_valid_waves(a.occWaves) ? a.occWaves : fd_default_waves(wpr);
+            const int nw = fd_valid_waves(a.occWaves) ? a.occWaves : fd_default_waves(wpr, a.chunkSlots);
             const int c = fd_occ_class(wpr, nw);
             switch (c) {
                 case 0: k_occlude<uint32_t, false, 2><<<chunks, 64 * nw, shm, st>>>(ao); break;

@@ -301,7 +301,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     FdLight L;
     light_setup(r->light, L);
     // layout of this request's class: the tuning table, else the defaults
-    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width, waves = fd_default_waves(L.wpr);
+    int bins = s->cfg.bins_per_side, tw = s->cfg.tile_width, waves = 0;   // 0: the default, once the frame is laid out
     int order = FD_TRI_ORDER_AUTO, split = 0;
     for (const fd_tune_entry& t : s->tune)
         if ((t.scene < 0 || t.scene == r->scene) && t.width == r->width && t.height == r->height &&
@@ -319,7 +319,7 @@ long long fd_server_submit(fd_server* s, const fd_request* r) {
     if ((1 << g) != bins || g < 3 || g > 6 || (1 << tl) != tw || tl < 3 || tl > 6) return FD_ERR_ARG;
     fill_args(q.args, md, (const float4*)r->d_position4, (const float4*)r->d_normal4, r->width,
               r->height, L, s->cfg.eps, s->cfg.backface_cull, g, tl, r->d_shadow, r->d_masks, s->cfg.max_pairs);
-    q.args.occWaves = waves;
+    q.args.occWaves = waves ? waves : fd_default_waves(L.wpr, q.args.chunkSlots);
     q.args.triOrder = order;
     apply_split(q.args, split);
     FdArgs probe = q.args;

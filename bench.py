@@ -194,7 +194,8 @@ def plain_leg(pkg, frame, workload, steps, warmup, quick):
                config=dict(workload=workload, receivers=f"{W}x{H}", light_samples=f"{R}x{R}",
                            triangles=frame.scene.ntris, bins=f"{1 << frame.grid_log2}x{1 << frame.grid_log2}",
                            tile_width=frame.tile_width,
-                           occ_waves=frame.occ_waves or pkg.autotune.default_waves(frame.light.res_w),
+                           occ_waves=frame.occ_waves or pkg.autotune.default_waves(frame.light.res_w, frame.rx.w,
+                                                                                   frame.rx.h, 1 << frame.grid_log2),
                            path="plain per-frame"),
                p50_ms=round(pctl(g["lat"], 50), 3), p99_ms=round(pctl(g["lat"], 99), 3),
                stage_ms=dict(zip(["light_plane", "receiver_binning", "triangle_pairing", "tables", "occlusion"],
@@ -490,7 +491,8 @@ def gpu_served(pkg, nreq=160, rate=500.0, budget_gb=48):
                 autotune=dict(seconds=round(t_tune, 2), classes=len(cache),
                               non_default=sum(1 for k, v in cache.items()
                                               if (v["bins"], v["tile"], v.get("waves"))
-                                              != pkg.autotune.default_layout(int(k.split(":")[2]))),
+                                              != pkg.autotune.default_layout(int(k.split(":")[2]),
+                                                                             *map(int, k.split(":")[1].split("x")))),
                               split_classes=sum(1 for v in cache.values() if v.get("split")),
                               orders={o: sum(1 for v in cache.values() if v.get("order") == i)
                                       for i, o in ((1, "list"), (2, "area"), (3, "mesh"))}))
@@ -509,7 +511,8 @@ def gpu_target_config(pkg, steps=10):
                 max_pairs=64 << 20)
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
-    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.default_layout(frame.light.res_w)
+    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.default_layout(frame.light.res_w, frame.rx.w,
+                                                                                   frame.rx.h)
     run.close()
     frame.grid_log2 = bins.bit_length() - 1
     frame.tile_width = tile
@@ -606,7 +609,8 @@ def gpu_shard_tile(pkg, steps=10):
                 max_pairs=64 << 20)
     cands = pkg.autotune.CANDIDATES
     best, _ = run.ctx.autotune(item, cands, 3)
-    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.default_layout(frame.light.res_w)
+    bins, tile, waves = cands[best] if best >= 0 else pkg.autotune.default_layout(frame.light.res_w, frame.rx.w,
+                                                                                   frame.rx.h)
     run.close()
     frame.grid_log2 = bins.bit_length() - 1
     frame.tile_width = tile

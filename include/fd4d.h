@@ -169,8 +169,9 @@ int fd_set_triangle_order(fd_ctx* ctx, int order);
 
 /* No counterpart (a schedule knob of this implementation): waves per work
  * item of the per-sample occlusion (64 receivers of one bin, whose triangle
- * list the waves split): 2, 4 or 8; 0 (the default) selects 4 for light
- * grids up to 32 x 32 and 8 above.  Results do not depend on it.  The
+ * list the waves split): 2, 4 or 8; 0 (the default) selects 8 for light
+ * grids above 32 x 32 and for frames of up to 8192 work items (about
+ * 710 x 710 receivers at 16 x 16 bins), 4 otherwise.  Results do not depend on it.  The
  * batched form takes it per instance (fd_instance.occ_waves), the serving
  * form per class (fd_tune_entry.occ_waves). */
 int fd_set_occlusion_waves(fd_ctx* ctx, int waves);

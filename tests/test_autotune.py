@@ -7,6 +7,11 @@ def test_cache_and_tables(tmp_path):
     at = importlib.import_module("4dzb_amd.autotune")
     assert at.default_layout(32) in at.CANDIDATES and at.default_layout(64) in at.CANDIDATES
     assert at.default_layout(32) == (16, 8, 4) and at.default_layout(33) == (16, 8, 8)
+    # frames of up to 8192 work items default to 8 waves, larger ones with 32-bit rows to 4
+    assert at.default_layout(32, 512, 512) == (16, 8, 8) and at.default_layout(32, 1024, 1024) == (16, 8, 4)
+    assert at.work_items(704, 704) <= at.WAVES8_ITEMS < at.work_items(720, 720)
+    assert at.default_waves(16, 704, 704) == 8 and at.default_waves(16, 720, 720) == 4
+    assert at.default_waves(64, 4096, 4096) == 8
     assert len(at.CANDIDATES) == len(at.BINS) * len(at.TILES) * len(at.WAVES)
     cache = {at.class_key(0, 512, 512, 32): dict(bins=32, tile=16, waves=2, order=at.ORDER_AREA, ms=1.5, default_ms=2.0,
                                                  candidates={}),
@@ -17,7 +22,8 @@ def test_cache_and_tables(tmp_path):
     assert at.load(str(tmp_path / "missing.json")) == {}
     assert at.layout(cache, 0, 512, 512, 32) == (32, 16, 2)
     assert at.layout(cache, 2, 1920, 1080, 16) == (8, 8, 4)   # an entry without waves: 4
-    assert at.layout(cache, 0, 256, 256, 32) == (16, 8, 4)   # unseen class: default layout
+    assert at.layout(cache, 0, 256, 256, 32) == (16, 8, 8)   # unseen class: default layout (a small frame)
+    assert at.layout(cache, 0, 1024, 1024, 32) == (16, 8, 4)
     assert at.layout(cache, 0, 256, 256, 64) == (16, 8, 8)
     assert at.order(cache, 0, 512, 512, 32) == at.ORDER_AREA
     assert at.order(cache, 2, 1920, 1080, 16) == 0   # an entry without an order: automatic

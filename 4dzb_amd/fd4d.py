@@ -337,8 +337,9 @@ class Context:
         _check(lib().fd_set_occlusion_variant(self.h, int(variant)), f"fd_set_occlusion_variant({variant})")
 
     def set_occlusion_waves(self, waves):
-        """Waves per occlusion work item: 2, 4 or 8; 0 = the default (4 for light
-        grids up to 32x32, 8 above).  Changes no result."""
+        """Waves per occlusion work item: 2, 4 or 8; 0 = the default (8 for light
+        grids above 32x32 and for frames of up to 8192 work items, else 4).
+        Changes no result."""
         _check(lib().fd_set_occlusion_waves(self.h, int(waves)), f"fd_set_occlusion_waves({waves})")
 
     def set_list_split(self, segs):

@@ -791,7 +791,8 @@ def main():
     ap.add_argument("--bins", type=int, default=0, help="bins per light-plane side (default: the workload's, 16)")
     ap.add_argument("--tile", type=int, default=0, help="receiver tile width (default 8)")
     ap.add_argument("--waves", type=int, default=0, choices=[0, 2, 4, 8],
-                    help="waves per occlusion work item (default: 4 up to 32x32 light samples, 8 above)")
+                    help="waves per occlusion work item (default: 8 above 32x32 light samples and for frames of "
+                         "up to 8192 work items, else 4)")
     ap.add_argument("--leg", choices=["batched", "served", "target", "app_pipeline", "shard_tile"], default=None,
                     help="run only this leg and print its JSON (for profiler runs)")
     args = ap.parse_args()

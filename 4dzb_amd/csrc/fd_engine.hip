@@ -163,12 +163,12 @@ void fill_args(FdArgs& a, const Model& md, const float4* pos, const float4* nrm,
 }
 
 // Stretches per bin list: req (1..FD_LIST_SPLIT_MAX), or for 0 the
-// automatic rule.  Small frames (up to FD_SPLIT_ITEMS work items) leave
-// most of the GPU idle while each work item walks its bin's whole list:
-// their lists split into about FD_SPLIT_WAVES waves in all.  Larger frames
-// fill the GPU already, and splitting would only cost the merge and the
-// early exit on full rows.  A frame without triangles has no lists to
-// split.  Needs a.occWaves.
+// automatic rule.  Small frames (up to FD_SPLIT_ITEMS work items and fewer
+// than FD_SPLIT_MIN_WAVES waves) leave most of the GPU idle while each work
+// item walks its bin's whole list: their lists split into about
+// FD_SPLIT_WAVES waves in all.  Larger frames fill the GPU already, and
+// splitting would only cost the merge and the early exit on full rows.  A
+// frame without triangles has no lists to split.  Needs a.occWaves.
 void apply_split(FdArgs& a, int req) {
     a.splitReq = req > 0 ? req : 0;
     if (a.ntris <= 0) {
@@ -177,7 +177,8 @@ void apply_split(FdArgs& a, int req) {
         a.segs = std::min(req, FD_LIST_SPLIT_MAX);
     } else {
         const long waves = (long)a.chunkSlots * std::max(a.occWaves, 1);
-        a.segs = a.chunkSlots <= FD_SPLIT_ITEMS ? (int)std::max(1L, std::min(16L, FD_SPLIT_WAVES / waves)) : 1;
+        a.segs = (a.chunkSlots <= FD_SPLIT_ITEMS && waves < FD_SPLIT_MIN_WAVES)
+                     ? (int)std::max(1L, std::min(16L, FD_SPLIT_WAVES / waves)) : 1;
     }
 }
 

@@ -91,13 +91,15 @@ struct FdArgs {
 };
 
 // Stretches of a bin's triangle list (fd_set_list_split): 1 = whole lists,
-// up to FD_LIST_SPLIT_MAX (fd4d.h).  The automatic rule: frames of up to
-// FD_SPLIT_ITEMS work items split their lists into about FD_SPLIT_WAVES
-// waves in all (work items x waves per item), at most 16 stretches.  Each
-// stretch merges its masks into the output by atomic OR, so the more mask
-// words per receiver, the less splitting pays.
+// up to FD_LIST_SPLIT_MAX (fd4d.h).  The automatic rule: a frame of at
+// most FD_SPLIT_ITEMS work items and fewer than FD_SPLIT_MIN_WAVES waves in
+// all (work items x waves per item) splits its lists into about
+// FD_SPLIT_WAVES waves, at most 16 stretches; larger frames keep whole
+// lists.  Each stretch merges its masks into the output by atomic OR, which
+// only pays where the GPU would otherwise idle.
 #define FD_SPLIT_ITEMS 2048
-#define FD_SPLIT_WAVES 20480
+#define FD_SPLIT_MIN_WAVES 6144
+#define FD_SPLIT_WAVES 49152
 
 // Waves per occlusion work item (per-sample kernel) unless a frame asks for
 // another count (2, 4 or 8): the waves share a work item's receivers, LDS

@@ -182,9 +182,11 @@ int fd_set_occlusion_waves(fd_ctx* ctx, int waves);
  * gives them S work items, one per stretch of the list, whose masks merge by
  * atomic OR into the (zeroed) mask output before a pass counts them.  OR
  * does not depend on the order, so the bits are the same for every S.
- * 0 (the default): frames of up to 2048 work items split into about 16384
- * work items (at most 16 stretches), larger ones do not (nor does a batch
- * whose instances total more than 2048); 1..64: that many stretches.  The
+ * 0 (the default): a frame of up to 2048 work items with fewer than 6144
+ * waves in all (work items x waves per item) splits into about 49152
+ * waves (at most 16 stretches); larger ones do not (nor does a batch whose
+ * instances total more than 2048 work items); 1..64: that many
+ * stretches.  The
  * batched form takes it per instance (fd_instance.list_split), the serving
  * form per class (fd_tune_entry.list_split). */
 #define FD_LIST_SPLIT_MAX 64
